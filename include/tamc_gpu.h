@@ -1,0 +1,219 @@
+/*
+ * tamc_gpu.h -- C ABI of the MI355X (gfx950) sampling engine for tamc's hot path:
+ * Metropolis sweeps over sparse couplings, parallel-tempering replica exchange, Houdayer
+ * isoenergetic cluster moves (ICM) and simulated annealing.
+ *
+ * The reference (USCqserver/tamc) has no FFI: its boundary is the `tamc` CLI plus the Rust runner
+ * API.  Each entry point below names the reference item it replaces (paths relative to that tree).
+ * All entry points take plain pointers and sizes, return a tamc_status, never throw or abort, and
+ * report details through tamc_last_error() (thread-local).  Inputs are copied; results are
+ * allocated by the library and released with the matching *_free function.
+ *
+ * Spins are int8 (+1/-1).  Bit-packed states use the reference's convention: bit set <=> spin -1,
+ * least significant bit first.
+ */
+#ifndef TAMC_GPU_H
+#define TAMC_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  TAMC_OK = 0,
+  TAMC_E_INVALID_ARG = 1, /* reference: panics on bad beta order, missing lo_beta with icm, ... */
+  TAMC_E_IO = 2,          /* reference: "Unable to read adjancency from instance file" */
+  TAMC_E_DEVICE = 3,      /* HIP runtime error or no gfx950 device */
+  TAMC_E_NOMEM = 4,
+  TAMC_E_UNSUPPORTED = 5, /* shape outside what the kernels accept (see tamc_last_error) */
+  TAMC_E_STREAM = 6       /* lock-step uniform stream exhausted */
+} tamc_status;
+
+/* Last error message of the calling thread ("" if none). */
+const char *tamc_last_error(void);
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int tamc_version(void);
+/* Initialise the HIP device (done lazily by every entry point that needs it). */
+tamc_status tamc_init(void);
+
+/* ------------------------------------------------------------------ instances ---- */
+typedef struct tamc_instance tamc_instance;
+
+/* BqmIsingInstance::from_instance_file (src/ising.rs:286-324) over read_adjacency_list
+ * (src/util.rs:69-109): "i j K" lines, quirks included (exponents truncated, duplicates overwrite). */
+tamc_status tamc_instance_from_file(const char *path, int qubo, tamc_instance **out);
+/* Same semantics from in-memory lines (i[k], j[k], K[k]) in file order. */
+tamc_status tamc_instance_from_lines(uint64_t nlines, const uint32_t *i, const uint32_t *j, const double *K,
+                                     int qubo, tamc_instance **out);
+/* BqmIsingInstance::with_suscept (src/ising.rs:388-403): nchi vectors of n coefficients (row-major). */
+tamc_status tamc_instance_set_suscept(tamc_instance *inst, uint32_t nchi, const double *w);
+/* read_txt_vec (src/util.rs:112-139) on one file, appended as one more suscept vector.  Like the
+ * reference, a length mismatch is reported on stdout and the vector kept; runs then fail. */
+tamc_status tamc_instance_add_suscept_file(tamc_instance *inst, const char *path);
+
+typedef struct {
+  uint32_t n;          /* BqmIsingInstance::size */
+  uint32_t nnz;        /* entries of coupling_vecs (QUBO self entries included) */
+  uint32_t graph_nnz;  /* nonzeros of the coupling CsMat (ICM graph) */
+  uint32_t graph_nodes;/* petgraph Csr node count = max index + 1 */
+  uint32_t nchi;
+  float offset;
+  int32_t qubo;
+} tamc_instance_info;
+tamc_status tamc_instance_get_info(const tamc_instance *inst, tamc_instance_info *info);
+/* Copy out coupling_vecs in CSR form: row_ptr[n+1], col[nnz], J[nnz], bias[n]. */
+tamc_status tamc_instance_get_csr(const tamc_instance *inst, uint32_t *row_ptr, uint32_t *col, float *J,
+                                  float *bias);
+void tamc_instance_destroy(tamc_instance *inst);
+
+/* --------------------------------------------- elementary device functions ---- */
+/* Instance::energy_ref (src/ising.rs:438-454) of nstates states [nstates][n], reference order. */
+tamc_status tamc_energy(const tamc_instance *inst, uint32_t nstates, const int8_t *states, float *energies);
+/* Instance::delta_energy (src/ising.rs:468-480) of every site of every state: out[nstates][n]. */
+tamc_status tamc_delta_energy(const tamc_instance *inst, uint32_t nstates, const int8_t *states, float *out);
+/* IsingState::overlap (src/ising.rs:159-165) of state pairs (a[k], b[k]). */
+tamc_status tamc_overlap(uint32_t n, uint32_t npairs, const int8_t *a, const int8_t *b, int64_t *q);
+/* IsingState::write_to_bytes (src/ising.rs:96-121): out[nstates][ceil(n/8)]. */
+tamc_status tamc_pack_bytes(uint32_t n, uint32_t nstates, const int8_t *states, uint8_t *out);
+/* houdayer_cluster_move (src/pt.rs:31-77) with the seed given as index k[p] into the ascending list
+ * of q<0 sites of pair p.  s1/s2 [npairs][n] are updated in place; cluster_size[p] = 0 when the pair
+ * has no q<0 site; clusters (nullable) [npairs][n] receives 1 for flipped sites. */
+tamc_status tamc_houdayer(const tamc_instance *inst, uint32_t npairs, int8_t *s1, int8_t *s2, const uint32_t *k,
+                          int64_t *cluster_size, uint8_t *clusters);
+
+/* --------------------------------------------------------------- PT-ICM ---- */
+/* PtIcmParams (src/pt.rs:259-272) with BetaOptions and lo_beta resolved by tamc_pt_resolve. */
+typedef struct {
+  uint32_t num_sweeps;
+  double warmup_fraction;
+  uint32_t num_betas;
+  const float *betas;       /* non-decreasing (PtIcmRunner::new panics otherwise) */
+  uint32_t lo_beta_idx;     /* first beta slot that takes part in ICM */
+  int32_t icm;
+  uint32_t num_replica_chains;
+  uint32_t threads;         /* >1 selects parallel_pt_loop semantics: stale exchange dE */
+  int32_t sample_enabled;   /* Option<u32> sample */
+  uint32_t sample;
+  int32_t sample_states_enabled;
+  uint32_t sample_states;
+  uint32_t sample_limiting; /* Option<u8>, 0 when None */
+  uint64_t seed;            /* production Philox key; results depend only on (instance, params, seed) */
+} tamc_pt_params;
+
+/* PtIcmRunner::new (src/pt.rs:307-351): geometric betas (tamc-core/src/sa.rs:13-22, cast to f32) or an
+ * explicit array, the non-decreasing check and lo_beta_idx.  lo_num_beta < 0 and lo_beta NaN mean None.
+ * betas_out must hold num_beta (geometric) or narr (array) floats. */
+tamc_status tamc_pt_resolve(int32_t geometric, float beta_min, float beta_max, uint32_t num_beta, const float *arr,
+                            uint32_t narr, float lo_beta, int64_t lo_num_beta, int32_t icm, float *betas_out,
+                            uint32_t *num_betas_out, uint32_t *lo_beta_idx_out);
+
+/* PtIcmMinResults + PtIcmThermalSamples + final Vec<PTState> (src/pt.rs:96-227, tamc-core/src/pt.rs:18-25). */
+typedef struct {
+  uint32_t n, B, C, T, meas_init, nchi, npairs;
+  uint32_t n_meas;             /* measurement events */
+  float *e;                    /* [B][n_meas*C]            PtIcmThermalSamples.e */
+  int32_t *q;                  /* [B][n_meas*npairs]       PtIcmThermalSamples.q */
+  float *suscept;              /* [B][nchi][n_meas*npairs] PtIcmThermalSamples.suscept */
+  uint32_t n_samp_slots, n_samp_events, nbytes;
+  uint8_t *samples;            /* [n_samp_slots][n_samp_events*C][nbytes] */
+  uint32_t n_gs, gs_words;
+  uint32_t *gs_steps;          /* MinResults.gs_time_steps */
+  float *gs_energies;          /* MinResults.gs_energies */
+  uint64_t *gs_states;         /* [n_gs][gs_words] MinResults.gs_states */
+  uint32_t *acceptance_counts; /* [B] count_acc (src/pt.rs:585-591) */
+  int8_t *final_states;        /* [C][B][n] slot order */
+  float *final_energies;       /* [C][B] tracked energies */
+  uint32_t *final_ids;         /* [C][B] physical replica id (chain*B + initial slot) */
+  uint32_t *num_acceptances;   /* [C][B] */
+  uint32_t *round_trips;       /* [C] */
+  uint8_t *tags;               /* [C][B] PTRoundTrip ordinal: None, MinBeta, MaxBeta, MinBetaRefl, MaxBetaRefl */
+  uint32_t *diffusion_hist;    /* [C][B][2] */
+  uint64_t icm_moves;
+  uint64_t attempts;           /* Metropolis spin-flip attempts performed */
+  double timing_us;            /* sweep loop time, as MinResults.timing (src/pt.rs:445-454) */
+  double device_ms;            /* HIP-event time of the kernels */
+} tamc_pt_result;
+
+/* PtIcmRunner::run / run_parallel (src/pt.rs:354-412): production path.  Sweeps in an order that
+ * is equivalent to the reference's sequential site order, Philox uniforms keyed by
+ * (seed, sweep, site quad, replica), exchange and ICM on the device.
+ * init_states (nullable) [C][B][n] replaces generate_init_state (src/pt.rs:506-519). */
+tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params, const int8_t *init_states,
+                        tamc_pt_result **out);
+
+/* Lock-step correctness mode: the same loop, but every random number comes from host-supplied u32
+ * word streams consumed in the reference's order (rand 0.8 conversions on the device).
+ * threads > 1: num_streams = C*B, stream c*B+b feeds slot b of chain c (exchange on stream c*B,
+ *              ICM choices on stream 0) -- parallel_pt_loop, src/pt.rs:414-457.
+ * threads <= 1: one stream for everything, in pt_loop order (src/pt.rs:459-504). */
+tamc_status tamc_pt_run_lockstep(const tamc_instance *inst, const tamc_pt_params *params, const int8_t *init_states,
+                                 uint32_t num_streams, const uint32_t *const *stream_words,
+                                 const uint64_t *stream_lens, tamc_pt_result **out);
+void tamc_pt_result_free(tamc_pt_result *r);
+
+/* ------------------------------------------------------------ batched PT ---- */
+typedef struct {
+  uint32_t waves_per_group;  /* 0 = autotune */
+  uint32_t sweeps_per_launch;/* 0 = autotune (all sweeps in one launch when 0 and not autotuning) */
+  int32_t use_graph;         /* capture the launch sequence in a hipGraph */
+  int32_t autotune;          /* time candidate tilings on a short prefix and keep the fastest */
+} tamc_batch_options;
+
+typedef struct {
+  double device_ms;          /* HIP-event time of all sweep launches */
+  double wall_ms;
+  uint64_t attempts;
+  uint32_t waves_per_group, sweeps_per_launch, launches, groups;
+} tamc_batch_stats;
+
+/* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is
+ * identical to tamc_pt_run on the same inputs.  results[count] receives one result per solve. */
+tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                              const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats);
+/* The plain path: the same solves, one instance per kernel launch, one after another. */
+tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                              tamc_pt_result **results, tamc_batch_stats *stats);
+
+/* ---------------------------------------------------------------------- SA ---- */
+/* SaParams (src/sa.rs:21-26) with BetaOptions resolved to the schedule array. */
+typedef struct {
+  uint32_t num_betas;
+  const float *betas;   /* one sweep per entry, non-decreasing (src/sa.rs:101-104) */
+  uint32_t num_replicas;
+  uint32_t threads;     /* num_replicas > 1: replicas = threads*(num_replicas/threads) (src/sa.rs:147,161) */
+  uint64_t seed;
+} tamc_sa_params;
+
+/* AnnealMinResults + final states (src/sa.rs:28-65, 110-165). */
+typedef struct {
+  uint32_t n, R;
+  float min_energy;
+  float *energies;      /* [R] energy_ref of the final states, reference order */
+  int8_t *final_states; /* [R][n] */
+  uint64_t attempts;
+  double timing_us;
+  double device_ms;
+} tamc_sa_result;
+
+/* SaRunner::run / run_parallel: sweeps in the reference's sequential site order, one replica per
+ * lane, Philox uniforms keyed by (seed, sweep, site quad, replica). */
+tamc_status tamc_sa_run(const tamc_instance *inst, const tamc_sa_params *params, tamc_sa_result **out);
+/* count independent SA solves in one launch. */
+tamc_status tamc_sa_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_sa_params *params,
+                              tamc_sa_result **results, tamc_batch_stats *stats);
+void tamc_sa_result_free(tamc_sa_result *r);
+
+/* --------------------------------------------------------------- outputs ---- */
+/* bincode-1 serialisation of PtIcmThermalSamples (src/pt.rs:116-125, written at src/pt.rs:652). */
+tamc_status tamc_write_thermal_samples(const tamc_pt_result *r, const float *betas, uint8_t compression_level,
+                                       const char *path);
+/* bincode-1 serialisation of AnnealState (src/sa.rs:67-89) for the states whose energy is within
+ * 1e-8 of the minimum (src/sa.rs:255-270). */
+tamc_status tamc_write_anneal_state(const tamc_sa_result *r, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

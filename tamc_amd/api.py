@@ -1,0 +1,338 @@
+"""Python mirror of the reference's runner API over the C ABI.
+
+Names follow the reference: ``BqmIsingInstance`` (src/ising.rs:257-324), ``BetaOptions`` and
+``PtIcmParams`` (src/pt.rs:229-290), ``PtIcmRunner`` (src/pt.rs:298-593), ``SaParams`` /
+``SaRunner`` (src/sa.rs:21-211).  Results are dictionaries of numpy arrays in the reference's
+layouts (``e[b][t*C + c]``, ``q[b][t*npairs + p]``, ...).  Every computation runs in
+libtamc_gpu.so on the GPU; nothing here computes a result itself.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from ._lib import (BatchOptions, BatchStats, InstanceInfo, PtParams as _CPt, PtResult, SaParams as _CSa, SaResult,
+                   check, lib, ptr, _f32p, _f64p, _i8p, _i64p, _u32p, _u64p, _u8p)
+
+
+class BqmIsingInstance:
+    """Ising/QUBO instance held by the library (BqmIsingInstance, src/ising.rs:257-324)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        info = InstanceInfo()
+        check(lib().tamc_instance_get_info(self._h, C.byref(info)))
+        self.n, self.nnz, self.offset, self.qubo = info.n, info.nnz, info.offset, bool(info.qubo)
+        self.graph_nodes = info.graph_nodes
+
+    @classmethod
+    def from_instance_file(cls, path, qubo=False):
+        h = C.c_void_p()
+        check(lib().tamc_instance_from_file(path.encode(), int(qubo), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_lines(cls, i, j, K, qubo=False):
+        i = np.ascontiguousarray(i, np.uint32)
+        j = np.ascontiguousarray(j, np.uint32)
+        K = np.ascontiguousarray(K, np.float64)
+        h = C.c_void_p()
+        check(lib().tamc_instance_from_lines(len(i), ptr(i, _u32p), ptr(j, _u32p), ptr(K, _f64p), int(qubo),
+                                             C.byref(h)))
+        return cls(h.value)
+
+    def with_suscept(self, w):
+        w = np.ascontiguousarray(w, np.float64).reshape(-1, self.n)
+        check(lib().tamc_instance_set_suscept(self._h, w.shape[0], ptr(w, _f64p)))
+        return self
+
+    def add_suscept_file(self, path):
+        check(lib().tamc_instance_add_suscept_file(self._h, path.encode()))
+
+    def size(self):
+        return self.n
+
+    def csr(self):
+        row_ptr = np.zeros(self.n + 1, np.uint32)
+        col = np.zeros(max(self.nnz, 1), np.uint32)
+        J = np.zeros(max(self.nnz, 1), np.float32)
+        bias = np.zeros(max(self.n, 1), np.float32)
+        check(lib().tamc_instance_get_csr(self._h, ptr(row_ptr, _u32p), ptr(col, _u32p), ptr(J, _f32p),
+                                          ptr(bias, _f32p)))
+        return row_ptr, col[:self.nnz], J[:self.nnz], bias[:self.n]
+
+    # -- parity surface ------------------------------------------------------------------
+    def energy(self, states):
+        s = np.ascontiguousarray(np.atleast_2d(states), np.int8)
+        out = np.zeros(s.shape[0], np.float32)
+        check(lib().tamc_energy(self._h, s.shape[0], ptr(s, _i8p), ptr(out, _f32p)))
+        return out
+
+    def delta_energy(self, states):
+        s = np.ascontiguousarray(np.atleast_2d(states), np.int8)
+        out = np.zeros(s.shape, np.float32)
+        check(lib().tamc_delta_energy(self._h, s.shape[0], ptr(s, _i8p), ptr(out, _f32p)))
+        return out
+
+    def houdayer(self, s1, s2, k):
+        """Houdayer move on state pairs with seeds k (index into the ascending q<0 list)."""
+        s1 = np.ascontiguousarray(np.atleast_2d(s1), np.int8).copy()
+        s2 = np.ascontiguousarray(np.atleast_2d(s2), np.int8).copy()
+        k = np.ascontiguousarray(np.atleast_1d(k), np.uint32)
+        sizes = np.zeros(s1.shape[0], np.int64)
+        cl = np.zeros(s1.shape, np.uint8)
+        check(lib().tamc_houdayer(self._h, s1.shape[0], ptr(s1, _i8p), ptr(s2, _i8p), ptr(k, _u32p),
+                                  ptr(sizes, _i64p), ptr(cl, _u8p)))
+        return s1, s2, sizes, cl.astype(bool)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().tamc_instance_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+def overlap(a, b):
+    a = np.ascontiguousarray(np.atleast_2d(a), np.int8)
+    b = np.ascontiguousarray(np.atleast_2d(b), np.int8)
+    q = np.zeros(a.shape[0], np.int64)
+    check(lib().tamc_overlap(a.shape[1], a.shape[0], ptr(a, _i8p), ptr(b, _i8p), ptr(q, _i64p)))
+    return q
+
+
+def pack_bytes(states):
+    s = np.ascontiguousarray(np.atleast_2d(states), np.int8)
+    out = np.zeros((s.shape[0], (s.shape[1] + 7) // 8), np.uint8)
+    check(lib().tamc_pack_bytes(s.shape[1], s.shape[0], ptr(s, _i8p), ptr(out, _u8p)))
+    return out
+
+
+@dataclass
+class BetaOptions:
+    """BetaOptions (src/pt.rs:229-257): Geometric(beta_min, beta_max, num_beta) or Arr(values)."""
+    geometric: bool = True
+    beta_min: float = 0.1
+    beta_max: float = 10.0
+    num_beta: int = 8
+    arr: Optional[List[float]] = None
+
+    @classmethod
+    def Geometric(cls, beta_min, beta_max, num_beta):
+        return cls(True, beta_min, beta_max, num_beta)
+
+    @classmethod
+    def Arr(cls, values):
+        return cls(False, arr=list(values))
+
+
+@dataclass
+class PtIcmParams:
+    """PtIcmParams (src/pt.rs:259-290) plus the production RNG seed."""
+    num_sweeps: int = 256
+    warmup_fraction: float = 0.5
+    beta: BetaOptions = field(default_factory=lambda: BetaOptions.Geometric(0.1, 10.0, 8))
+    lo_beta: Optional[float] = None
+    lo_num_beta: Optional[int] = None
+    icm: bool = True
+    num_replica_chains: int = 2
+    threads: int = 1
+    sample: Optional[int] = 32
+    sample_states: Optional[int] = 64
+    sample_limiting: Optional[int] = 0
+    seed: int = 0
+
+    def resolve(self):
+        """PtIcmRunner::new (src/pt.rs:307-351): (betas f32 array, lo_beta_idx)."""
+        b = self.beta
+        nb_cap = b.num_beta if b.geometric else len(b.arr)
+        out = np.zeros(max(nb_cap, 1), np.float32)
+        arr = np.ascontiguousarray(b.arr if b.arr is not None else [0.0], np.float32)
+        nb = C.c_uint32()
+        lo = C.c_uint32()
+        check(lib().tamc_pt_resolve(int(b.geometric), C.c_float(b.beta_min), C.c_float(b.beta_max), b.num_beta,
+                                    ptr(arr, _f32p), len(b.arr) if b.arr is not None else 0,
+                                    C.c_float(float("nan") if self.lo_beta is None else self.lo_beta),
+                                    -1 if self.lo_num_beta is None else int(self.lo_num_beta), int(self.icm),
+                                    ptr(out, _f32p), C.byref(nb), C.byref(lo)))
+        return out[:nb.value].copy(), lo.value
+
+    def to_c(self, betas, lo):
+        P = _CPt()
+        P.num_sweeps = self.num_sweeps
+        P.warmup_fraction = self.warmup_fraction
+        P.num_betas = len(betas)
+        P.betas = ptr(betas, _f32p)
+        P.lo_beta_idx = lo
+        P.icm = int(self.icm)
+        P.num_replica_chains = self.num_replica_chains
+        P.threads = self.threads
+        P.sample_enabled = int(self.sample is not None)
+        P.sample = self.sample or 0
+        P.sample_states_enabled = int(self.sample_states is not None)
+        P.sample_states = self.sample_states or 0
+        P.sample_limiting = self.sample_limiting or 0
+        P.seed = self.seed
+        return P
+
+
+def _arr(p, count, dtype):
+    if count == 0:
+        return np.zeros(0, dtype)
+    return np.ctypeslib.as_array(p, shape=(count,)).copy().astype(dtype)
+
+
+def pt_result_to_dict(R):
+    n, B, Cc, npairs, nm = R.n, R.B, R.C, R.npairs, R.n_meas
+    return dict(
+        n=n, B=B, C=Cc, T=R.T, meas_init=R.meas_init, n_meas=nm,
+        e=_arr(R.e, B * nm * Cc, np.float32).reshape(B, nm * Cc),
+        q=_arr(R.q, B * nm * npairs, np.int32).reshape(B, nm * npairs),
+        suscept=_arr(R.suscept, B * R.nchi * nm * npairs, np.float32).reshape(B, R.nchi, nm * npairs),
+        samples=_arr(R.samples, R.n_samp_slots * R.n_samp_events * Cc * R.nbytes, np.uint8).reshape(
+            R.n_samp_slots, R.n_samp_events * Cc, R.nbytes),
+        gs_steps=_arr(R.gs_steps, R.n_gs, np.uint32),
+        gs_energies=_arr(R.gs_energies, R.n_gs, np.float32),
+        gs_states=_arr(R.gs_states, R.n_gs * R.gs_words, np.uint64).reshape(R.n_gs, R.gs_words),
+        acceptance_counts=_arr(R.acceptance_counts, B, np.uint32),
+        final_states=_arr(R.final_states, Cc * B * n, np.int8).reshape(Cc, B, n),
+        final_energies=_arr(R.final_energies, Cc * B, np.float32).reshape(Cc, B),
+        final_ids=_arr(R.final_ids, Cc * B, np.uint32).reshape(Cc, B),
+        num_acceptances=_arr(R.num_acceptances, Cc * B, np.uint32).reshape(Cc, B),
+        round_trips=_arr(R.round_trips, Cc, np.uint32),
+        tags=_arr(R.tags, Cc * B, np.uint8).reshape(Cc, B),
+        diffusion_hist=_arr(R.diffusion_hist, Cc * B * 2, np.uint32).reshape(Cc, B, 2),
+        icm_moves=R.icm_moves, attempts=R.attempts, timing_us=R.timing_us, device_ms=R.device_ms)
+
+
+class PtIcmRunner:
+    """PtIcmRunner (src/pt.rs:298-593) on the GPU."""
+
+    def __init__(self, instance, params):
+        self.instance = instance
+        self.params = params
+        self.betas, self.lo_beta_idx = params.resolve()
+
+    def _c(self):
+        return self.params.to_c(self.betas, self.lo_beta_idx)
+
+    def run(self, init_states=None):
+        """Production path; returns the result dictionary."""
+        P = self._c()
+        out = C.POINTER(PtResult)()
+        init = None if init_states is None else np.ascontiguousarray(init_states, np.int8)
+        check(lib().tamc_pt_run(self.instance._h, C.byref(P), None if init is None else ptr(init, _i8p),
+                                C.byref(out)))
+        try:
+            return pt_result_to_dict(out.contents)
+        finally:
+            lib().tamc_pt_result_free(out)
+
+    run_parallel = run
+
+    def run_lockstep(self, init_states, streams):
+        """Lock-step mode: every random number from the given u32 word streams (see the C header)."""
+        P = self._c()
+        init = np.ascontiguousarray(init_states, np.int8)
+        arrs = [np.ascontiguousarray(s, np.uint32) for s in streams]
+        ptrs = (_u32p * len(arrs))(*[ptr(a, _u32p) for a in arrs])
+        lens = np.array([len(a) for a in arrs], np.uint64)
+        out = C.POINTER(PtResult)()
+        check(lib().tamc_pt_run_lockstep(self.instance._h, C.byref(P), ptr(init, _i8p), len(arrs), ptrs,
+                                         ptr(lens, _u64p), C.byref(out)))
+        try:
+            return pt_result_to_dict(out.contents)
+        finally:
+            lib().tamc_pt_result_free(out)
+
+    def write_thermal_samples(self, result_ptr, path):
+        raise NotImplementedError
+
+
+def _batch_args(instances, params_list):
+    resolved = [p.resolve() for p in params_list]
+    cps = (_CPt * len(instances))()
+    for k, (p, (b, lo)) in enumerate(zip(params_list, resolved)):
+        cps[k] = p.to_c(b, lo)
+    hs = (C.c_void_p * len(instances))(*[inst._h.value for inst in instances])
+    return resolved, cps, hs
+
+
+def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False):
+    """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path."""
+    resolved, cps, hs = _batch_args(instances, params_list)
+    outs = (C.POINTER(PtResult) * len(instances))()
+    stats = BatchStats()
+    if plain:
+        check(lib().tamc_pt_run_plain(len(instances), hs, cps, outs, C.byref(stats)))
+    else:
+        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), 0)
+        check(lib().tamc_pt_run_batch(len(instances), hs, cps, C.byref(opt), outs, C.byref(stats)))
+    res = []
+    try:
+        for k in range(len(instances)):
+            res.append(pt_result_to_dict(outs[k].contents))
+    finally:
+        for k in range(len(instances)):
+            lib().tamc_pt_result_free(outs[k])
+    st = dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts,
+              waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
+              launches=stats.launches, groups=stats.groups)
+    return res, st
+
+
+@dataclass
+class SaParams:
+    """SaParams (src/sa.rs:21-26) plus the production RNG seed."""
+    beta: BetaOptions = field(default_factory=lambda: BetaOptions.Geometric(0.1, 10.0, 256))
+    num_replicas: int = 1
+    threads: int = 1
+    seed: int = 0
+
+    def betas(self):
+        b = self.beta
+        if b.geometric:
+            p = PtIcmParams(beta=b, icm=False)
+            return p.resolve()[0]
+        return np.ascontiguousarray(b.arr, np.float32)
+
+
+def _sa_dict(R):
+    return dict(n=R.n, R=R.R, min_energy=R.min_energy, energies=_arr(R.energies, R.R, np.float32),
+                final_states=_arr(R.final_states, R.R * R.n, np.int8).reshape(R.R, R.n), attempts=R.attempts,
+                timing_us=R.timing_us, device_ms=R.device_ms)
+
+
+def run_sa_batch(instances, params_list):
+    keep = [p.betas() for p in params_list]
+    cps = (_CSa * len(instances))()
+    for k, (p, b) in enumerate(zip(params_list, keep)):
+        cps[k] = _CSa(len(b), ptr(b, _f32p), p.num_replicas, p.threads, p.seed)
+    hs = (C.c_void_p * len(instances))(*[inst._h.value for inst in instances])
+    outs = (C.POINTER(SaResult) * len(instances))()
+    stats = BatchStats()
+    check(lib().tamc_sa_run_batch(len(instances), hs, cps, outs, C.byref(stats)))
+    res = []
+    try:
+        for k in range(len(instances)):
+            res.append(_sa_dict(outs[k].contents))
+    finally:
+        for k in range(len(instances)):
+            lib().tamc_sa_result_free(outs[k])
+    return res, dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts)
+
+
+class SaRunner:
+    """SaRunner (src/sa.rs:90-211) on the GPU."""
+
+    def __init__(self, instance, params):
+        self.instance = instance
+        self.params = params
+
+    def run(self):
+        return run_sa_batch([self.instance], [self.params])[0][0]
+
+    run_parallel = run

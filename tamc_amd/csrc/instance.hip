@@ -1,0 +1,337 @@
+// instance.hip -- error state, device initialisation and tamc_instance: the reference's instance
+// file rules (src/util.rs:43-109), the Ising/QUBO build (src/ising.rs:286-324), device upload and
+// the quad-level schedule that makes parallel sweeps equal to the sequential site order.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <string>
+
+#include "runtime_common.hpp"
+
+namespace tamc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+tamc_status fail(tamc_status st, const std::string &msg) {
+  g_last_error = msg;
+  return st;
+}
+
+tamc_status device_init() {
+  static std::once_flag once;
+  static tamc_status st = TAMC_OK;
+  static std::string msg;
+  std::call_once(once, []() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+      st = TAMC_E_DEVICE;
+      msg = "no HIP device visible";
+      return;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+      st = TAMC_E_DEVICE;
+      msg = "hipGetDeviceProperties failed";
+      return;
+    }
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) {
+      st = TAMC_E_DEVICE;
+      msg = std::string("kernels are built for gfx950, device is ") + prop.gcnArchName;
+      return;
+    }
+    if (hipSetDevice(0) != hipSuccess) {
+      st = TAMC_E_DEVICE;
+      msg = "hipSetDevice(0) failed";
+    }
+  });
+  if (st != TAMC_OK) return fail(st, msg);
+  return TAMC_OK;
+}
+
+// ------------------------------------------------------------------ parsing ----
+// One line of an instance file: space0 u32 space1 u32 space1 fixed, remainder ignored.
+static bool parse_instance_line(const char *s, size_t len, uint32_t *oi, uint32_t *oj, double *ok) {
+  size_t p = 0;
+  uint32_t v[2];
+  for (int f = 0; f < 2; ++f) {
+    while (p < len && (s[p] == ' ' || s[p] == '\t')) ++p;
+    if (p >= len || s[p] < '0' || s[p] > '9') return false;
+    uint64_t x = 0;
+    while (p < len && s[p] >= '0' && s[p] <= '9') {
+      x = x * 10u + (uint64_t)(s[p] - '0');
+      if (x > 0xFFFFFFFFull) return false;
+      ++p;
+    }
+    v[f] = (uint32_t)x;
+    if (p >= len || (s[p] != ' ' && s[p] != '\t')) return false;
+    while (p < len && (s[p] == ' ' || s[p] == '\t')) ++p;
+  }
+  const size_t start = p;
+  if (p < len && s[p] == '-') ++p;
+  const size_t dig = p;
+  while (p < len && s[p] >= '0' && s[p] <= '9') ++p;
+  if (p == dig) return false;
+  if (p < len && s[p] == '.') {
+    ++p;
+    while (p < len && s[p] >= '0' && s[p] <= '9') ++p;
+  }
+  std::string num(s + start, p - start);
+  *ok = strtod(num.c_str(), nullptr);
+  *oi = v[0];
+  *oj = v[1];
+  return true;
+}
+
+using Adj = std::vector<std::map<uint32_t, float>>;
+
+static void adj_put(Adj &adj, uint32_t i, uint32_t j, double K, bool verbose) {
+  const size_t m = (size_t)(i > j ? i : j) + 1;
+  if (adj.size() < m) adj.resize(m);
+  const float k = (float)K;
+  auto it = adj[i].find(j);
+  if (it != adj[i].end() && verbose) printf("The entry (%u, %u) :> %g was overwritten with %g.\n", i, j, it->second, K);
+  adj[i][j] = k;
+  if (i != j) {
+    auto it2 = adj[j].find(i);
+    if (it2 != adj[j].end() && verbose)
+      printf("The entry (%u, %u) :> %g was overwritten with %g.\n", i, j, it2->second, K);
+    adj[j][i] = k;
+  }
+}
+
+static void compute_levels(tamc_instance *I) {
+  const uint32_t nq = (I->n + 3u) / 4u;
+  I->quad_level.assign(nq, 0u);
+  uint32_t maxl = 0;
+  for (uint32_t q = 0; q < nq; ++q) {
+    uint32_t lv = 0;
+    for (uint32_t i = 4u * q; i < std::min(I->n, 4u * q + 4u); ++i)
+      for (uint32_t e = I->row_ptr[i]; e < I->row_ptr[i + 1]; ++e) {
+        const uint32_t qj = I->col[e] / 4u;
+        if (qj < q) lv = std::max(lv, I->quad_level[qj] + 1u);
+      }
+    I->quad_level[q] = lv;
+    maxl = std::max(maxl, lv);
+  }
+  I->n_levels = nq ? maxl + 1u : 0u;
+}
+
+static tamc_instance *build(const Adj &adj, int qubo) {
+  tamc_instance *I = new tamc_instance();
+  const uint32_t n = (uint32_t)adj.size();
+  I->n = n;
+  I->qubo = qubo;
+  I->bias.assign(n, 0.f);
+  I->row_ptr.assign(n + 1, 0u);
+  float offset = 0.f;
+  std::vector<std::pair<uint32_t, uint32_t>> pat;
+  for (uint32_t i = 0; i < n; ++i) {
+    for (const auto &kv : adj[i]) {
+      const uint32_t j = kv.first;
+      const float K = kv.second;
+      if (qubo) {
+        if (i != j) {
+          offset += K / 8.0f;
+          pat.emplace_back(i, j);
+          I->col.push_back(j);
+          I->J.push_back(K / 4.0f);
+          I->bias[i] += K / 4.0f;
+        } else {
+          offset += K / 2.0f;
+          I->bias[i] += K / 2.0f;
+          I->col.push_back(j);
+          I->J.push_back(K / 2.0f);
+        }
+      } else if (i != j) {
+        pat.emplace_back(i, j);
+        I->col.push_back(j);
+        I->J.push_back(K);
+      } else {
+        I->bias[i] = K;
+      }
+    }
+    I->row_ptr[i + 1] = (uint32_t)I->col.size();
+  }
+  I->offset = offset;
+  uint32_t gn = 0;
+  for (auto &e : pat) gn = std::max(gn, std::max(e.first, e.second) + 1u);
+  I->g_nodes = pat.empty() ? 0u : gn;
+  I->g_row_ptr.assign((size_t)I->g_nodes + 1, 0u);
+  for (auto &e : pat) I->g_row_ptr[e.first + 1]++;
+  for (uint32_t r = 0; r < I->g_nodes; ++r) I->g_row_ptr[r + 1] += I->g_row_ptr[r];
+  I->g_col.resize(pat.size());
+  std::vector<uint32_t> fill(I->g_row_ptr.begin(), I->g_row_ptr.end());
+  for (auto &e : pat) I->g_col[fill[e.first]++] = e.second;
+  compute_levels(I);
+  return I;
+}
+
+DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_status *st) {
+  tamc_instance *inst = const_cast<tamc_instance *>(cinst);
+  std::lock_guard<std::mutex> lk(inst->mu);
+  *st = TAMC_OK;
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> bool {
+    if (b.p) return true;
+    if (b.alloc(bytes) != hipSuccess) return false;
+    if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return false;
+    return true;
+  };
+  if (!inst->dev) inst->dev.reset(new DeviceInstance());
+  DeviceInstance *D = inst->dev.get();
+  std::vector<double> chi;
+  for (auto &v : inst->chi) chi.insert(chi.end(), v.begin(), v.end());
+  if (!up(D->row_ptr, inst->row_ptr.data(), inst->row_ptr.size() * 4) ||
+      !up(D->col, inst->col.data(), inst->col.size() * 4) || !up(D->J, inst->J.data(), inst->J.size() * 4) ||
+      !up(D->bias, inst->bias.data(), inst->bias.size() * 4) ||
+      !up(D->g_row_ptr, inst->g_row_ptr.data(), inst->g_row_ptr.size() * 4) ||
+      !up(D->g_col, inst->g_col.data(), inst->g_col.size() * 4) || !up(D->chi, chi.data(), chi.size() * 8)) {
+    *st = fail(TAMC_E_DEVICE, "instance upload failed");
+    return nullptr;
+  }
+  if (!D->sched.count(G)) {
+    std::unique_ptr<DeviceSchedule> S(new DeviceSchedule());
+    const uint32_t nq = (uint32_t)inst->quad_level.size();
+    std::vector<uint32_t> cnt(inst->n_levels + 1, 0u);
+    for (uint32_t q = 0; q < nq; ++q) cnt[inst->quad_level[q] + 1] += G;
+    for (uint32_t l = 0; l < inst->n_levels; ++l) cnt[l + 1] += cnt[l];
+    std::vector<uint32_t> tasks(cnt.back());
+    std::vector<uint32_t> fill(cnt.begin(), cnt.end());
+    for (uint32_t q = 0; q < nq; ++q)
+      for (uint32_t g = 0; g < G; ++g) tasks[fill[inst->quad_level[q]]++] = q * G + g;
+    S->n_levels = inst->n_levels;
+    S->n_tasks = (uint32_t)tasks.size();
+    if (!up(S->lvl_off, cnt.data(), cnt.size() * 4) || !up(S->tasks, tasks.data(), tasks.size() * 4)) {
+      *st = fail(TAMC_E_DEVICE, "schedule upload failed");
+      return nullptr;
+    }
+    D->sched[G] = std::move(S);
+  }
+  return D;
+}
+
+const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st) {
+  DeviceInstance *D = device_instance(inst, G, st);
+  if (!D) return nullptr;
+  std::lock_guard<std::mutex> lk(const_cast<tamc_instance *>(inst)->mu);
+  return D->sched[G].get();
+}
+
+}  // namespace tamc
+
+using namespace tamc;
+
+extern "C" {
+
+const char *tamc_last_error(void) { return g_last_error.c_str(); }
+int tamc_version(void) { return 100; }
+tamc_status tamc_init(void) { return device_init(); }
+
+tamc_status tamc_instance_from_file(const char *path, int qubo, tamc_instance **out) {
+  clear_error();
+  if (!path || !out) return fail(TAMC_E_INVALID_ARG, "null argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(TAMC_E_IO, std::string("Unable to read adjancency from instance file ") + path);
+  Adj adj;
+  std::string line;
+  uint64_t ln = 0;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    uint32_t i, j;
+    double K;
+    if (parse_instance_line(line.data(), line.size(), &i, &j, &K)) adj_put(adj, i, j, K, true);
+    else printf("Ignoring line %llu: %s\n", (unsigned long long)ln, line.c_str());
+    ++ln;
+  }
+  *out = build(adj, qubo);
+  return TAMC_OK;
+}
+
+tamc_status tamc_instance_from_lines(uint64_t nlines, const uint32_t *i, const uint32_t *j, const double *K, int qubo,
+                                     tamc_instance **out) {
+  clear_error();
+  if (!out || (nlines && (!i || !j || !K))) return fail(TAMC_E_INVALID_ARG, "null argument");
+  Adj adj;
+  for (uint64_t k = 0; k < nlines; ++k) adj_put(adj, i[k], j[k], K[k], false);
+  *out = build(adj, qubo);
+  return TAMC_OK;
+}
+
+tamc_status tamc_instance_set_suscept(tamc_instance *inst, uint32_t nchi, const double *w) {
+  clear_error();
+  if (!inst || (nchi && !w)) return fail(TAMC_E_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(inst->mu);
+  inst->chi.assign(nchi, std::vector<double>(inst->n));
+  for (uint32_t k = 0; k < nchi; ++k) memcpy(inst->chi[k].data(), w + (size_t)k * inst->n, inst->n * sizeof(double));
+  if (inst->dev) inst->dev->chi.reset();
+  return TAMC_OK;
+}
+
+tamc_status tamc_instance_add_suscept_file(tamc_instance *inst, const char *path) {
+  clear_error();
+  if (!inst || !path) return fail(TAMC_E_INVALID_ARG, "null argument");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(TAMC_E_IO, std::string("Unable to open susceptibility file ") + path);
+  std::vector<double> v;
+  std::string line;
+  uint64_t ln = 0;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    // space0 fixed multispace0 (src/util.rs:117-121)
+    size_t p = 0;
+    while (p < line.size() && (line[p] == ' ' || line[p] == '\t')) ++p;
+    const size_t s = p;
+    if (p < line.size() && line[p] == '-') ++p;
+    const size_t d = p;
+    while (p < line.size() && isdigit((unsigned char)line[p])) ++p;
+    if (p == d) {
+      printf("Ignoring line %llu: %s\n", (unsigned long long)ln, line.c_str());
+    } else {
+      if (p < line.size() && line[p] == '.') {
+        ++p;
+        while (p < line.size() && isdigit((unsigned char)line[p])) ++p;
+      }
+      v.push_back(strtod(line.substr(s, p - s).c_str(), nullptr));
+    }
+    ++ln;
+  }
+  std::lock_guard<std::mutex> lk(inst->mu);
+  if (v.size() != inst->n)
+    printf("WARNING: Ignoring suscept file %s - Expected %u coefficients, but got %zu\n", path, inst->n, v.size());
+  inst->chi.push_back(v);
+  if (inst->dev) inst->dev->chi.reset();
+  return TAMC_OK;
+}
+
+tamc_status tamc_instance_get_info(const tamc_instance *inst, tamc_instance_info *info) {
+  clear_error();
+  if (!inst || !info) return fail(TAMC_E_INVALID_ARG, "null argument");
+  info->n = inst->n;
+  info->nnz = (uint32_t)inst->col.size();
+  info->graph_nnz = (uint32_t)inst->g_col.size();
+  info->graph_nodes = inst->g_nodes;
+  info->nchi = (uint32_t)inst->chi.size();
+  info->offset = inst->offset;
+  info->qubo = inst->qubo;
+  return TAMC_OK;
+}
+
+tamc_status tamc_instance_get_csr(const tamc_instance *inst, uint32_t *row_ptr, uint32_t *col, float *J, float *bias) {
+  clear_error();
+  if (!inst) return fail(TAMC_E_INVALID_ARG, "null argument");
+  if (row_ptr) memcpy(row_ptr, inst->row_ptr.data(), inst->row_ptr.size() * 4);
+  if (col && !inst->col.empty()) memcpy(col, inst->col.data(), inst->col.size() * 4);
+  if (J && !inst->J.empty()) memcpy(J, inst->J.data(), inst->J.size() * 4);
+  if (bias && inst->n) memcpy(bias, inst->bias.data(), inst->n * 4);
+  return TAMC_OK;
+}
+
+void tamc_instance_destroy(tamc_instance *inst) { delete inst; }
+
+}  // extern "C"

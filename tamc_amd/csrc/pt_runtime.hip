@@ -1,0 +1,676 @@
+// pt_runtime.hip -- host side of the production PT-ICM path: PtIcmRunner::new resolution
+// (src/pt.rs:307-351), group planning, device buffers, kernel launches and the assembly of
+// PtIcmMinResults / PtIcmThermalSamples / final PTState (src/pt.rs:96-227, 542-591).
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "pt_types.hpp"
+#include "runtime_common.hpp"
+
+extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t ngroups,
+                                          const uint32_t *const *init_bits, hipStream_t s);
+extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
+                                            uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, hipStream_t s);
+
+namespace tamc {
+
+constexpr uint32_t kLdsLimit = 160u * 1024u;
+
+template <class T>
+static T *host_dup(const std::vector<T> &v) {
+  T *p = (T *)malloc(std::max<size_t>(1, v.size() * sizeof(T)));
+  if (!v.empty()) memcpy(p, v.data(), v.size() * sizeof(T));
+  return p;
+}
+
+struct GroupPlan {
+  uint32_t chain0, nch, R, G, P;
+  bool icm_global = false;
+  PtGroup g{};  // device pointers filled at allocation
+  std::vector<uint32_t> init_bits;
+};
+
+struct SolvePlan {
+  const tamc_instance *inst = nullptr;
+  tamc_pt_params P{};
+  std::vector<float> betas;
+  const int8_t *init_states = nullptr;
+  uint32_t n = 0, B = 0, C = 0, T = 0, meas_init = 0, n_meas = 0, n_samp_events = 0, n_samp_slots = 0, nbytes = 0,
+           gs_cap = 0, gs_words = 0, npairs = 0, nchi = 0;
+  std::vector<GroupPlan> groups;
+  float *d_betas = nullptr;
+};
+
+static uint32_t count_events(uint32_t T, uint32_t meas_init, uint32_t S) {
+  if (T <= meas_init || S == 0) return 0;
+  const uint32_t last = T - 1 - meas_init;
+  return last / S + 1 + (last % S ? 1 : 0);
+}
+
+static tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_params *P,
+                              const int8_t *init_states, uint32_t waves) {
+  if (!inst || !P) return fail(TAMC_E_INVALID_ARG, "null argument");
+  if (P->num_betas < 1 || !P->betas) return fail(TAMC_E_INVALID_ARG, "num_betas must be >= 1");
+  if (P->num_replica_chains < 1) return fail(TAMC_E_INVALID_ARG, "num_replica_chains must be >= 1");
+  for (uint32_t b = 1; b < P->num_betas; ++b)
+    if (!(P->betas[b] - P->betas[b - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+  if (P->lo_beta_idx > P->num_betas) return fail(TAMC_E_INVALID_ARG, "lo_beta_idx out of range");
+  if ((P->sample_enabled && P->sample == 0) || (P->sample_states_enabled && P->sample_states == 0))
+    return fail(TAMC_E_INVALID_ARG, "sample intervals must be positive");
+  if (inst->n == 0) return fail(TAMC_E_INVALID_ARG, "empty instance");
+  for (auto &v : inst->chi)
+    if (v.size() != inst->n) return fail(TAMC_E_INVALID_ARG, "suscept vector length differs from the instance size");
+  S.inst = inst;
+  S.P = *P;
+  S.betas.assign(P->betas, P->betas + P->num_betas);
+  S.P.betas = S.betas.data();
+  S.init_states = init_states;
+  S.n = inst->n;
+  S.B = P->num_betas;
+  S.C = P->num_replica_chains;
+  S.T = P->num_sweeps;
+  const double mi = P->warmup_fraction * (double)P->num_sweeps;  // `as u32` saturates
+  S.meas_init = (mi != mi || mi <= 0.0) ? 0u : (mi >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)mi);
+  S.n_meas = P->sample_enabled ? count_events(S.T, S.meas_init, P->sample) : 0;
+  S.n_samp_events = P->sample_states_enabled ? count_events(S.T, S.meas_init, P->sample_states) : 0;
+  S.n_samp_slots = P->sample_limiting == 0 ? S.B : (P->sample_limiting == 1 ? S.B / 2 : 1);
+  S.nbytes = (S.n + 7) / 8;
+  S.gs_words = (S.n + 63) / 64;
+  S.gs_cap = S.T > S.meas_init ? S.T - S.meas_init : 0;
+  S.npairs = S.C / 2;
+  S.nchi = (uint32_t)inst->chi.size();
+  for (uint32_t c0 = 0; c0 < S.C; c0 += 2) {
+    GroupPlan g;
+    g.chain0 = c0;
+    g.nch = std::min(2u, S.C - c0);
+    g.R = g.nch * S.B;
+    if (g.R > kMaxGroupReplicas)
+      return fail(TAMC_E_UNSUPPORTED, "a chain pair may hold at most 256 replicas (num_betas <= 128)");
+    g.G = (g.R + 63) / 64;
+    g.P = (P->icm && g.nch == 2 && P->lo_beta_idx < S.B) ? S.B - P->lo_beta_idx : 0;
+    if (g.P > kMaxPairs) return fail(TAMC_E_UNSUPPORTED, "at most 32 beta slots may take part in ICM");
+    const LdsLayout L = make_layout(S.n, g.G, g.R, S.B, g.P, waves, true);
+    if (L.total > kLdsLimit) {
+      const LdsLayout L2 = make_layout(S.n, g.G, g.R, S.B, g.P, waves, false);
+      if (L2.total > kLdsLimit) return fail(TAMC_E_UNSUPPORTED, "instance too large for the LDS-resident kernel");
+      g.icm_global = true;
+    }
+    if (init_states) {
+      // bit planes [n][G][2]: bit l of half h of word g <=> replica 64g+32h+l is -1
+      g.init_bits.assign((size_t)S.n * g.G * 2, 0u);
+      for (uint32_t r = 0; r < g.R; ++r) {
+        const uint32_t c = g.chain0 + r / S.B, b = r % S.B;
+        const int8_t *s = init_states + ((size_t)c * S.B + b) * S.n;
+        for (uint32_t i = 0; i < S.n; ++i)
+          if (!(s[i] > 0)) g.init_bits[((size_t)i * g.G + (r >> 6)) * 2 + ((r >> 5) & 1)] |= 1u << (r & 31);
+      }
+    }
+    S.groups.push_back(std::move(g));
+  }
+  return TAMC_OK;
+}
+
+static uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
+  return make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global).total;
+}
+
+// Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
+static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
+  for (auto &S : solves) {
+    A.want(&S.d_betas, S.B);
+    for (auto &g : S.groups) {
+      PtGroup &X = g.g;
+      const uint32_t nb = (S.n + 63) / 64;
+      A.want(&X.spins, (size_t)S.n * g.G * 2);
+      A.want(&X.energy, g.R);
+      A.want(&X.perm, (size_t)g.nch * S.B);
+      A.want(&X.tags, (size_t)g.nch * S.B);
+      A.want(&X.nacc, (size_t)g.nch * S.B);
+      A.want(&X.hist, (size_t)g.nch * S.B * 2);
+      A.want(&X.rt, g.nch);
+      A.want(&X.gs_min, 1);
+      A.want(&X.gs_count, 1);
+      A.want(&X.icm_moves, 1);
+      A.want(&X.out_e, (size_t)S.n_meas * g.nch * S.B);
+      A.want(&X.out_q, (size_t)S.n_meas * S.B);
+      A.want(&X.out_chi, (size_t)S.n_meas * S.B * S.nchi);
+      A.want(&X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes);
+      A.want(&X.out_gs_steps, S.gs_cap);
+      A.want(&X.out_gs_e, S.gs_cap);
+      A.want(&X.out_gs_states, (size_t)S.gs_cap * S.gs_words);
+      if (g.icm_global) {
+        A.want(&X.g_qw, S.n);
+        A.want(&X.g_cw, S.n);
+        A.want(&X.g_blk, (size_t)nb * g.P);
+      }
+    }
+  }
+  TAMC_HIP(A.commit());
+  for (auto &S : solves) {
+    TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
+    for (auto &g : S.groups) {
+      tamc_status st;
+      DeviceInstance *D = device_instance(S.inst, g.G, &st);
+      if (!D) return st;
+      const DeviceSchedule *Sc = device_schedule(S.inst, g.G, &st);
+      if (!Sc) return st;
+      PtGroup &X = g.g;
+      X.row_ptr = (const uint32_t *)D->row_ptr.p;
+      X.col = (const uint32_t *)D->col.p;
+      X.J = (const float *)D->J.p;
+      X.bias = (const float *)D->bias.p;
+      X.g_row_ptr = (const uint32_t *)D->g_row_ptr.p;
+      X.g_col = (const uint32_t *)D->g_col.p;
+      X.chi = (const double *)D->chi.p;
+      X.offset = S.inst->offset;
+      X.n = S.n;
+      X.g_nodes = S.inst->g_nodes;
+      X.nchi = S.nchi;
+      X.nblk = (S.n + 63) / 64;
+      X.lvl_off = (const uint32_t *)Sc->lvl_off.p;
+      X.tasks = (const uint32_t *)Sc->tasks.p;
+      X.n_levels = Sc->n_levels;
+      X.betas = S.d_betas;
+      X.B = S.B;
+      X.nch = g.nch;
+      X.chain0 = g.chain0;
+      X.lo = S.P.lo_beta_idx;
+      X.G = g.G;
+      X.R = g.R;
+      X.P = g.P;
+      X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
+                (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
+                (g.icm_global ? kFlagIcmGlobal : 0u);
+      X.T = S.T;
+      X.meas_init = S.meas_init;
+      X.sample = S.P.sample_enabled ? S.P.sample : 1u;
+      X.sample_states = S.P.sample_states_enabled ? S.P.sample_states : 1u;
+      X.sample_limiting = S.P.sample_limiting;
+      X.n_samp_slots = S.n_samp_slots;
+      X.gs_cap = S.gs_cap;
+      X.nbytes = S.nbytes;
+      X.gs_words = S.gs_words;
+      X.pair_global = g.chain0 / 2;
+      X.seed = S.P.seed;
+    }
+  }
+  return TAMC_OK;
+}
+
+struct Launch {
+  std::vector<PtGroup> order;        // groups in launch order (largest first)
+  std::vector<const uint32_t *> init_ptrs;
+  DevBuf d_groups, d_init_ptrs;
+  std::vector<DevBuf *> init_bufs;
+  uint32_t lds = 0, icm_lds = 1, max_T = 0;
+  ~Launch() {
+    for (auto *b : init_bufs) delete b;
+  }
+};
+
+static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, Launch &Lc) {
+  struct Item {
+    SolvePlan *S;
+    GroupPlan *g;
+    double cost;
+  };
+  std::vector<Item> items;
+  bool any_init = false;
+  for (auto &S : solves)
+    for (auto &g : S.groups) {
+      items.push_back({&S, &g, (double)S.n * g.R * S.T});
+      any_init |= !g.init_bits.empty();
+      Lc.lds = std::max(Lc.lds, lds_bytes_for(S, g, waves));
+      Lc.max_T = std::max(Lc.max_T, S.T);
+    }
+  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+  for (auto &it : items) {
+    Lc.order.push_back(it.g->g);
+    if (any_init) {
+      if (it.g->init_bits.empty()) {
+        Lc.init_ptrs.push_back(nullptr);
+      } else {
+        DevBuf *b = new DevBuf();
+        Lc.init_bufs.push_back(b);
+        TAMC_HIP(b->alloc(it.g->init_bits.size() * 4));
+        TAMC_HIP(hipMemcpy(b->p, it.g->init_bits.data(), it.g->init_bits.size() * 4, hipMemcpyHostToDevice));
+        Lc.init_ptrs.push_back((const uint32_t *)b->p);
+      }
+    }
+  }
+  TAMC_HIP(Lc.d_groups.alloc(Lc.order.size() * sizeof(PtGroup)));
+  TAMC_HIP(hipMemcpy(Lc.d_groups.p, Lc.order.data(), Lc.order.size() * sizeof(PtGroup), hipMemcpyHostToDevice));
+  if (any_init) {
+    TAMC_HIP(Lc.d_init_ptrs.alloc(Lc.init_ptrs.size() * sizeof(void *)));
+    TAMC_HIP(hipMemcpy(Lc.d_init_ptrs.p, Lc.init_ptrs.data(), Lc.init_ptrs.size() * sizeof(void *),
+                       hipMemcpyHostToDevice));
+  }
+  return TAMC_OK;
+}
+
+template <class T>
+static tamc_status d2h(std::vector<T> &v, const T *d, size_t count) {
+  v.resize(count);
+  if (count) TAMC_HIP(hipMemcpy(v.data(), d, count * sizeof(T), hipMemcpyDeviceToHost));
+  return TAMC_OK;
+}
+
+// Gather one solve's group outputs into a tamc_pt_result laid out like the reference structures.
+static tamc_status assemble(const SolvePlan &S, tamc_pt_result **out) {
+  const uint32_t n = S.n, B = S.B, C = S.C, nm = S.n_meas, np = S.npairs;
+  tamc_pt_result *R = (tamc_pt_result *)calloc(1, sizeof(tamc_pt_result));
+  R->n = n;
+  R->B = B;
+  R->C = C;
+  R->T = S.T;
+  R->meas_init = S.meas_init;
+  R->nchi = S.nchi;
+  R->npairs = np;
+  R->n_meas = nm;
+  R->n_samp_slots = S.n_samp_slots;
+  R->n_samp_events = S.n_samp_events;
+  R->nbytes = S.nbytes;
+  R->gs_words = S.gs_words;
+  std::vector<float> e((size_t)B * nm * C), chi((size_t)B * S.nchi * nm * np);
+  std::vector<int32_t> q((size_t)B * nm * np);
+  std::vector<uint8_t> smp((size_t)S.n_samp_slots * S.n_samp_events * C * S.nbytes);
+  std::vector<int8_t> fin((size_t)C * B * n);
+  std::vector<float> fe((size_t)C * B);
+  std::vector<uint32_t> ids((size_t)C * B), nacc((size_t)C * B), hist((size_t)C * B * 2), rts(C), accs(B, 0);
+  std::vector<uint8_t> tags((size_t)C * B);
+  struct GsRec {
+    uint32_t step;
+    float e;
+    uint32_t group;
+    std::vector<uint64_t> st;
+  };
+  std::vector<std::vector<GsRec>> gsr(S.groups.size());
+  uint64_t moves = 0;
+  tamc_status st;
+  for (size_t gi = 0; gi < S.groups.size(); ++gi) {
+    const GroupPlan &g = S.groups[gi];
+    const PtGroup &X = g.g;
+    std::vector<float> oe, ochi, energy, gsmin, gse;
+    std::vector<int32_t> oq;
+    std::vector<uint8_t> os, otags;
+    std::vector<uint32_t> spins, perm, onacc, ohist, ort, gsc, gss;
+    std::vector<uint64_t> gsst;
+    std::vector<unsigned long long> mv;
+    if ((st = d2h(oe, X.out_e, (size_t)nm * g.nch * B)) || (st = d2h(oq, X.out_q, (size_t)nm * B)) ||
+        (st = d2h(ochi, X.out_chi, (size_t)nm * B * S.nchi)) ||
+        (st = d2h(os, X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes)) ||
+        (st = d2h(spins, X.spins, (size_t)n * g.G * 2)) || (st = d2h(energy, X.energy, g.R)) ||
+        (st = d2h(perm, X.perm, (size_t)g.nch * B)) || (st = d2h(otags, X.tags, (size_t)g.nch * B)) ||
+        (st = d2h(onacc, X.nacc, (size_t)g.nch * B)) || (st = d2h(ohist, X.hist, (size_t)g.nch * B * 2)) ||
+        (st = d2h(ort, X.rt, g.nch)) || (st = d2h(gsc, X.gs_count, 1)) || (st = d2h(mv, X.icm_moves, 1))) {
+      free(R);
+      return st;
+    }
+    moves += mv[0];
+    const uint32_t ngs = std::min(gsc[0], S.gs_cap);
+    if ((st = d2h(gss, X.out_gs_steps, ngs)) || (st = d2h(gse, X.out_gs_e, ngs)) ||
+        (st = d2h(gsst, X.out_gs_states, (size_t)ngs * S.gs_words))) {
+      free(R);
+      return st;
+    }
+    for (uint32_t k = 0; k < ngs; ++k)
+      gsr[gi].push_back(GsRec{gss[k], gse[k], (uint32_t)gi,
+                              std::vector<uint64_t>(gsst.begin() + (size_t)k * S.gs_words,
+                                                    gsst.begin() + (size_t)(k + 1) * S.gs_words)});
+    for (uint32_t ev = 0; ev < nm; ++ev)
+      for (uint32_t cl = 0; cl < g.nch; ++cl)
+        for (uint32_t b = 0; b < B; ++b)
+          e[(size_t)b * nm * C + (size_t)ev * C + g.chain0 + cl] = oe[((size_t)ev * g.nch + cl) * B + b];
+    if (g.nch == 2) {
+      const uint32_t p = g.chain0 / 2;
+      for (uint32_t ev = 0; ev < nm; ++ev)
+        for (uint32_t b = 0; b < B; ++b) {
+          q[(size_t)b * nm * np + (size_t)ev * np + p] = oq[(size_t)ev * B + b];
+          for (uint32_t k = 0; k < S.nchi; ++k)
+            chi[((size_t)b * S.nchi + k) * nm * np + (size_t)ev * np + p] = ochi[((size_t)ev * B + b) * S.nchi + k];
+        }
+    }
+    for (uint32_t sev = 0; sev < S.n_samp_events; ++sev)
+      for (uint32_t slot = 0; slot < S.n_samp_slots; ++slot)
+        for (uint32_t cl = 0; cl < g.nch; ++cl)
+          memcpy(&smp[(((size_t)slot * S.n_samp_events * C) + (size_t)sev * C + g.chain0 + cl) * S.nbytes],
+                 &os[(((size_t)sev * S.n_samp_slots + slot) * g.nch + cl) * S.nbytes], S.nbytes);
+    for (uint32_t cl = 0; cl < g.nch; ++cl) {
+      const uint32_t c = g.chain0 + cl;
+      for (uint32_t b = 0; b < B; ++b) {
+        const uint32_t r = perm[cl * B + b];
+        for (uint32_t i = 0; i < n; ++i) {
+          const uint32_t bit = (spins[((size_t)i * g.G + (r >> 6)) * 2 + ((r >> 5) & 1)] >> (r & 31)) & 1u;
+          fin[((size_t)c * B + b) * n + i] = bit ? -1 : 1;
+        }
+        fe[(size_t)c * B + b] = energy[r];
+        ids[(size_t)c * B + b] = g.chain0 * B + r;
+        nacc[(size_t)c * B + b] = onacc[cl * B + b];
+        accs[b] += onacc[cl * B + b];
+        tags[(size_t)c * B + b] = otags[cl * B + b];
+        hist[((size_t)c * B + b) * 2] = ohist[(cl * B + b) * 2];
+        hist[((size_t)c * B + b) * 2 + 1] = ohist[(cl * B + b) * 2 + 1];
+      }
+      rts[c] = ort[cl];
+    }
+  }
+  // Merge per-group ground-state records into the global trace (src/pt.rs:559-581): a global
+  // improvement is always an improvement of the group holding the minimum, and ties go to the
+  // lowest chain, i.e. the lowest group.
+  std::vector<uint32_t> gsteps;
+  std::vector<float> genergies;
+  std::vector<uint64_t> gstates;
+  {
+    std::vector<size_t> pos(gsr.size(), 0);
+    bool have = false;
+    float cur = 0.f;
+    for (;;) {
+      uint32_t tmin = 0xFFFFFFFFu;
+      for (size_t gi = 0; gi < gsr.size(); ++gi)
+        if (pos[gi] < gsr[gi].size()) tmin = std::min(tmin, gsr[gi][pos[gi]].step);
+      if (tmin == 0xFFFFFFFFu) break;
+      const GsRec *best = nullptr;
+      for (size_t gi = 0; gi < gsr.size(); ++gi)
+        if (pos[gi] < gsr[gi].size() && gsr[gi][pos[gi]].step == tmin) {
+          const GsRec &r = gsr[gi][pos[gi]];
+          if (!best || r.e < best->e) best = &r;
+          ++pos[gi];
+        }
+      if (!have || best->e < cur) {
+        have = true;
+        cur = best->e;
+        gsteps.push_back(best->step);
+        genergies.push_back(best->e);
+        gstates.insert(gstates.end(), best->st.begin(), best->st.end());
+      }
+    }
+  }
+  R->n_gs = (uint32_t)gsteps.size();
+  R->e = host_dup(e);
+  R->q = host_dup(q);
+  R->suscept = host_dup(chi);
+  R->samples = host_dup(smp);
+  R->gs_steps = host_dup(gsteps);
+  R->gs_energies = host_dup(genergies);
+  R->gs_states = host_dup(gstates);
+  R->acceptance_counts = host_dup(accs);
+  R->final_states = host_dup(fin);
+  R->final_energies = host_dup(fe);
+  R->final_ids = host_dup(ids);
+  R->num_acceptances = host_dup(nacc);
+  R->round_trips = host_dup(rts);
+  R->tags = host_dup(tags);
+  R->diffusion_hist = host_dup(hist);
+  R->icm_moves = moves;
+  R->attempts = (uint64_t)S.T * n * B * C;
+  *out = R;
+  return TAMC_OK;
+}
+
+struct RunOptions {
+  uint32_t waves = 8;
+  uint32_t sweeps_per_launch = 0;  // 0 = all
+  bool use_graph = false;
+};
+
+// Plan, allocate, launch and assemble a set of solves that run concurrently in one kernel.
+static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &opt, tamc_pt_result **results,
+                              tamc_batch_stats *stats, const std::function<void(std::vector<SolvePlan> &)> &patch = {}) {
+  Arena A;
+  tamc_status st = allocate(solves, A);
+  if (st) return st;
+  if (patch) patch(solves);
+  Launch Lc;
+  if ((st = prepare_launch(solves, opt.waves, Lc))) return st;
+  const uint32_t ng = (uint32_t)Lc.order.size();
+  hipStream_t s;
+  TAMC_HIP(hipStreamCreate(&s));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{s};
+  const PtGroup *dg = (const PtGroup *)Lc.d_groups.p;
+  TAMC_HIP(tamc_launch_pt_init(dg, ng, Lc.init_ptrs.empty() ? nullptr : (const uint32_t *const *)Lc.d_init_ptrs.p, s));
+  TAMC_HIP(hipStreamSynchronize(s));
+  hipEvent_t e0, e1;
+  TAMC_HIP(hipEventCreate(&e0));
+  TAMC_HIP(hipEventCreate(&e1));
+  const uint32_t spl = opt.sweeps_per_launch ? opt.sweeps_per_launch : std::max(1u, Lc.max_T);
+  uint32_t launches = 0;
+  const auto w0 = std::chrono::steady_clock::now();
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  TAMC_HIP(hipEventRecord(e0, s));
+  if (opt.use_graph) {
+    TAMC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
+      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, s);
+    TAMC_HIP(hipStreamEndCapture(s, &graph));
+    TAMC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    TAMC_HIP(hipGraphLaunch(gexec, s));
+  } else {
+    for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
+      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, s));
+  }
+  TAMC_HIP(hipEventRecord(e1, s));
+  TAMC_HIP(hipEventSynchronize(e1));
+  const auto w1 = std::chrono::steady_clock::now();
+  float ms = 0.f;
+  TAMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  const double wall_us = std::chrono::duration<double, std::micro>(w1 - w0).count();
+  uint64_t attempts = 0;
+  for (size_t k = 0; k < solves.size(); ++k) {
+    if ((st = assemble(solves[k], &results[k]))) {
+      for (size_t j = 0; j < k; ++j) tamc_pt_result_free(results[j]);
+      return st;
+    }
+    results[k]->timing_us = wall_us;
+    results[k]->device_ms = ms;
+    attempts += results[k]->attempts;
+  }
+  if (stats) {
+    stats->device_ms = ms;
+    stats->wall_ms = wall_us / 1000.0;
+    stats->attempts = attempts;
+    stats->waves_per_group = opt.waves;
+    stats->sweeps_per_launch = spl;
+    stats->launches = launches;
+    stats->groups = ng;
+  }
+  return TAMC_OK;
+}
+
+static uint32_t default_waves(const tamc_instance *inst) {
+  // enough waves to cover the typical level width, within one work group
+  const uint32_t nq = (uint32_t)inst->quad_level.size();
+  const uint32_t width = inst->n_levels ? (nq + inst->n_levels - 1) / inst->n_levels : 1;
+  uint32_t w = 4;
+  while (w < width && w < 16) w *= 2;
+  return w;
+}
+
+}  // namespace tamc
+
+using namespace tamc;
+
+extern "C" {
+
+tamc_status tamc_pt_resolve(int32_t geometric, float beta_min, float beta_max, uint32_t num_beta, const float *arr,
+                            uint32_t narr, float lo_beta, int64_t lo_num_beta, int32_t icm, float *betas_out,
+                            uint32_t *num_betas_out, uint32_t *lo_beta_idx_out) {
+  clear_error();
+  if (!betas_out || !num_betas_out || !lo_beta_idx_out) return fail(TAMC_E_INVALID_ARG, "null argument");
+  uint32_t nb;
+  if (geometric) {
+    // geometric_beta_schedule (tamc-core/src/sa.rs:13-22) in f64, then `as f32`
+    nb = num_beta;
+    const double l0 = std::log((double)beta_min), lf = std::log((double)beta_max);
+    const double d = (lf - l0) / (double)(nb - 1);
+    for (uint32_t i = 0; i < nb; ++i) betas_out[i] = (float)std::exp(l0 + d * (double)i);
+  } else {
+    nb = narr;
+    if (narr && !arr) return fail(TAMC_E_INVALID_ARG, "null beta array");
+    memcpy(betas_out, arr, narr * sizeof(float));
+  }
+  if (nb == 0) return fail(TAMC_E_INVALID_ARG, "empty beta array");
+  for (uint32_t b = 1; b < nb; ++b)
+    if (!(betas_out[b] - betas_out[b - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+  uint32_t lo;
+  if (lo_num_beta >= 0) {
+    const uint64_t ln = std::min<uint64_t>((uint64_t)lo_num_beta, nb);
+    lo = nb - (uint32_t)ln;
+  } else if (lo_beta == lo_beta) {
+    lo = nb - 1;  // out of bounds: the largest beta is assigned (src/pt.rs:328-331)
+    for (uint32_t b = 0; b < nb; ++b)
+      if (betas_out[b] >= lo_beta) {
+        lo = b;
+        break;
+      }
+  } else {
+    lo = 0;
+    if (icm) return fail(TAMC_E_INVALID_ARG, "Must specify either lo_beta or lo_num_beta for ICM");
+  }
+  *num_betas_out = nb;
+  *lo_beta_idx_out = lo;
+  return TAMC_OK;
+}
+
+tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params, const int8_t *init_states,
+                        tamc_pt_result **out) {
+  clear_error();
+  if (!out) return fail(TAMC_E_INVALID_ARG, "null result pointer");
+  tamc_status st = device_init();
+  if (st) return st;
+  std::vector<SolvePlan> solves(1);
+  RunOptions opt;
+  opt.waves = inst ? default_waves(inst) : 8;
+  if ((st = plan_solve(solves[0], inst, params, init_states, opt.waves))) return st;
+  return run_solves(solves, opt, out, nullptr);
+}
+
+tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                              const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats) {
+  clear_error();
+  if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
+  tamc_status st = device_init();
+  if (st) return st;
+  RunOptions opt;
+  uint32_t w = 4;
+  for (uint32_t k = 0; k < count; ++k) w = std::max(w, insts[k] ? default_waves(insts[k]) : 4u);
+  opt.waves = (opts && opts->waves_per_group) ? opts->waves_per_group : w;
+  if (opt.waves < 1 || opt.waves > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
+  opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
+  opt.use_graph = opts && opts->use_graph;
+  std::vector<SolvePlan> solves(count);
+  for (uint32_t k = 0; k < count; ++k)
+    if ((st = plan_solve(solves[k], insts[k], &params[k], nullptr, opt.waves))) return st;
+  return run_solves(solves, opt, results, stats);
+}
+
+tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                              tamc_pt_result **results, tamc_batch_stats *stats) {
+  clear_error();
+  if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
+  tamc_status st = device_init();
+  if (st) return st;
+  tamc_batch_stats acc{};
+  for (uint32_t k = 0; k < count; ++k) {
+    std::vector<SolvePlan> solves(1);
+    RunOptions opt;
+    opt.waves = insts[k] ? default_waves(insts[k]) : 8;
+    tamc_batch_stats one{};
+    if ((st = plan_solve(solves[0], insts[k], &params[k], nullptr, opt.waves)) ||
+        (st = run_solves(solves, opt, &results[k], &one))) {
+      for (uint32_t j = 0; j < k; ++j) tamc_pt_result_free(results[j]);
+      return st;
+    }
+    acc.device_ms += one.device_ms;
+    acc.wall_ms += one.wall_ms;
+    acc.attempts += one.attempts;
+    acc.launches += one.launches;
+    acc.groups += one.groups;
+  }
+  if (stats) *stats = acc;
+  return TAMC_OK;
+}
+
+tamc_status tamc_houdayer(const tamc_instance *inst, uint32_t npairs, int8_t *s1, int8_t *s2, const uint32_t *k,
+                          int64_t *cluster_size, uint8_t *clusters) {
+  clear_error();
+  if (!inst || (npairs && (!s1 || !s2 || !k || !cluster_size))) return fail(TAMC_E_INVALID_ARG, "null argument");
+  tamc_status st = device_init();
+  if (st) return st;
+  const uint32_t n = inst->n;
+  for (uint32_t p = 0; p < npairs; ++p) {
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < n; ++i) cnt += ((int)s1[(size_t)p * n + i] * (int)s2[(size_t)p * n + i] < 0);
+    if (cnt && k[p] >= cnt) return fail(TAMC_E_INVALID_ARG, "seed index beyond the number of q<0 sites");
+  }
+  for (uint32_t p0 = 0; p0 < npairs; p0 += kMaxPairs) {
+    const uint32_t np = std::min(kMaxPairs, npairs - p0);
+    std::vector<float> betas(np, 0.0f);
+    std::vector<int8_t> init((size_t)2 * np * n);
+    memcpy(init.data(), s1 + (size_t)p0 * n, (size_t)np * n);
+    memcpy(init.data() + (size_t)np * n, s2 + (size_t)p0 * n, (size_t)np * n);
+    tamc_pt_params P{};
+    P.num_sweeps = 1;
+    P.warmup_fraction = 1.0;
+    P.num_betas = np;
+    P.betas = betas.data();
+    P.lo_beta_idx = 0;
+    P.icm = 1;
+    P.num_replica_chains = 2;
+    P.threads = 1;
+    std::vector<SolvePlan> solves(1);
+    RunOptions opt;
+    opt.waves = default_waves(inst);
+    if ((st = plan_solve(solves[0], inst, &P, init.data(), opt.waves))) return st;
+    DevBuf dk;
+    TAMC_HIP(dk.alloc(np * 4));
+    TAMC_HIP(hipMemcpy(dk.p, k + p0, np * 4, hipMemcpyHostToDevice));
+    tamc_pt_result *R = nullptr;
+    st = run_solves(solves, opt, &R, nullptr, [&](std::vector<SolvePlan> &ss) {
+      ss[0].groups[0].g.flags |= kFlagIcmOnly;
+      ss[0].groups[0].g.icm_k = (const uint32_t *)dk.p;
+    });
+    if (st) return st;
+    for (uint32_t p = 0; p < np; ++p) {
+      int8_t *a = s1 + (size_t)(p0 + p) * n, *b = s2 + (size_t)(p0 + p) * n;
+      const int8_t *na = R->final_states + (size_t)p * n, *nb = R->final_states + ((size_t)np + p) * n;
+      int64_t sz = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        const bool ch = a[i] != na[i];
+        sz += ch;
+        if (clusters) clusters[(size_t)(p0 + p) * n + i] = ch ? 1 : 0;
+        a[i] = na[i];
+        b[i] = nb[i];
+      }
+      cluster_size[p0 + p] = sz;
+    }
+    tamc_pt_result_free(R);
+  }
+  return TAMC_OK;
+}
+
+void tamc_pt_result_free(tamc_pt_result *R) {
+  if (!R) return;
+  void *ptrs[] = {R->e, R->q, R->suscept, R->samples, R->gs_steps, R->gs_energies, R->gs_states,
+                  R->acceptance_counts, R->final_states, R->final_energies, R->final_ids, R->num_acceptances,
+                  R->round_trips, R->tags, R->diffusion_hist};
+  for (void *p : ptrs) free(p);
+  free(R);
+}
+
+}  // extern "C"

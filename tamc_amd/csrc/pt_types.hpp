@@ -1,0 +1,128 @@
+// pt_types.hpp -- layout of one PT-ICM work group ("group" = one instance x one ICM chain pair)
+// as seen by the gfx950 kernels, plus the LDS arena layout.  Shared by kernels.hip and runtime.cpp.
+//
+// Spin storage ("bit planes"): for every site i the group keeps G = ceil(R/64) 64-bit words, one bit
+// per replica (bit set <=> spin -1, the reference's packing convention).  A wavefront of 64 lanes
+// handles the 64 replicas of one word, so coupling rows are read once per 64 replicas and every
+// neighbour-spin read is a single broadcast LDS load.
+#pragma once
+#include <stdint.h>
+
+namespace tamc {
+
+constexpr uint32_t kMaxWaves = 16;       // waves per work group (1024 threads)
+constexpr uint32_t kMaxGroupReplicas = 256;
+constexpr uint32_t kMaxPairs = 32;       // ICM pairs per group held in one 32-bit cluster word
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+enum : uint32_t {
+  kFlagIcm = 1u,
+  kFlagThreaded = 2u,        // stale exchange dE (tamc-core/src/parallel/pt.rs:51-60)
+  kFlagSample = 4u,
+  kFlagSampleStates = 8u,
+  kFlagIcmGlobal = 16u,      // ICM scratch lives in global memory (group too large for LDS)
+  kFlagIcmOnly = 32u,        // run only the ICM step with seeds given in icm_k (tamc_houdayer)
+};
+
+struct PtGroup {
+  // ---- instance (device pointers, read only) ----
+  const uint32_t *row_ptr;   // [n+1] coupling_vecs
+  const uint32_t *col;       // [nnz]
+  const float *J;            // [nnz]
+  const float *bias;         // [n]
+  const uint32_t *g_row_ptr; // [g_nodes+1] coupling CsMat pattern (ICM graph)
+  const uint32_t *g_col;
+  const double *chi;         // [nchi][n]
+  float offset;
+  uint32_t n, g_nodes, nchi, nblk;
+  // ---- sequential-order schedule: levels of site quads ----
+  const uint32_t *lvl_off;   // [n_levels+1] offsets into tasks
+  const uint32_t *tasks;     // quad*G + g
+  uint32_t n_levels;
+  // ---- ladder ----
+  const float *betas;        // [B]
+  uint32_t B, nch, chain0, lo; // chains in group (1 or 2), first global chain, first ICM slot
+  uint32_t G, R, P, flags;     // words per site, replicas, ICM pairs, kFlag*
+  uint32_t T, meas_init, sample, sample_states;
+  uint32_t sample_limiting, n_samp_slots, gs_cap, nbytes;
+  uint32_t gs_words, pair_global, pad0, pad1;
+  uint64_t seed;
+  // ---- persistent state (global memory, carried between launches) ----
+  uint32_t *spins;           // [n][G][2] 32-bit halves of the bit-plane words
+  float *energy;             // [R] tracked energies per physical replica
+  uint32_t *perm;            // [nch][B] slot -> physical replica index in the group
+  uint8_t *tags;             // [nch][B] round-trip tags in slot order
+  uint32_t *nacc;            // [nch][B]
+  uint32_t *hist;            // [nch][B][2]
+  uint32_t *rt;              // [nch]
+  float *gs_min;             // [1]
+  uint32_t *gs_count;        // [1]
+  unsigned long long *icm_moves; // [1]
+  // ---- outputs ----
+  float *out_e;              // [n_meas][nch][B]
+  int32_t *out_q;            // [n_meas][B]
+  float *out_chi;            // [n_meas][B][nchi]
+  uint8_t *out_samples;      // [n_samp_events][n_samp_slots][nch][nbytes]
+  uint32_t *out_gs_steps;    // [gs_cap]
+  float *out_gs_e;           // [gs_cap]
+  uint64_t *out_gs_states;   // [gs_cap][gs_words]
+  // ---- ICM scratch when it does not fit in LDS ----
+  uint32_t *g_qw, *g_cw, *g_blk;
+  // ---- kFlagIcmOnly: seed index per pair (into the ascending q<0 list) ----
+  const uint32_t *icm_k;
+};
+
+struct LdsLayout {
+  uint32_t spins, energy, beta_of, perm, tags, pair_a, pair_b, pair_of, seed_blk, seed_rank, misc, partial, dE, qw,
+      cw, blk, total;
+};
+
+__host__ __device__ inline uint32_t lds_align(uint32_t x) { return (x + 15u) & ~15u; }
+
+// icm_lds: ICM arrays (q words, cluster words, per-block counts) are placed in LDS.
+__host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_t R, uint32_t B, uint32_t P,
+                                                 uint32_t W, bool icm_lds) {
+  LdsLayout L;
+  uint32_t o = 0;
+  const uint32_t nblk = (n + 63u) / 64u;
+  L.spins = o;
+  o += lds_align(n * G * 8u);
+  L.energy = o;
+  o += lds_align(R * 4u);
+  L.beta_of = o;
+  o += lds_align(R * 4u);
+  L.perm = o;
+  o += lds_align(2u * B * 4u);
+  L.tags = o;
+  o += lds_align(2u * B);
+  L.pair_a = o;
+  o += lds_align(kMaxPairs * 4u);
+  L.pair_b = o;
+  o += lds_align(kMaxPairs * 4u);
+  L.pair_of = o;
+  o += lds_align(R * 4u);
+  L.seed_blk = o;
+  o += lds_align(kMaxPairs * 4u);
+  L.seed_rank = o;
+  o += lds_align(kMaxPairs * 4u);
+  L.misc = o;
+  o += lds_align(16u * 4u);
+  L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G)
+  o += lds_align(W * G * 64u * 8u);
+  L.dE = o;  // exchange energy differences [2][B]
+  o += lds_align(2u * B * 4u);
+  if (icm_lds && P > 0) {
+    L.qw = o;
+    o += lds_align(n * 4u);
+    L.cw = o;
+    o += lds_align(n * 4u);
+    L.blk = o;
+    o += lds_align(nblk * P * 4u);
+  } else {
+    L.qw = L.cw = L.blk = kNone;
+  }
+  L.total = o;
+  return L;
+}
+
+}  // namespace tamc

@@ -1,0 +1,106 @@
+// runtime_common.hpp -- host-side structures of the runtime: error state, instances (parsed and
+// built with the reference's rules), device copies and the sequential-order quad schedule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tamc_gpu.h"
+
+namespace tamc {
+
+void set_error(const std::string &msg);
+void clear_error();
+tamc_status fail(tamc_status st, const std::string &msg);
+tamc_status device_init();
+
+#define TAMC_HIP(expr)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return ::tamc::fail(TAMC_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));    \
+  } while (0)
+
+// Device allocation that is freed on scope exit.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    reset();
+    bytes = b;
+    return hipMalloc(&p, b ? b : 16);
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// Bump allocator over one device allocation.
+struct Arena {
+  size_t used = 0;
+  std::vector<std::pair<void **, size_t>> reqs;
+  template <class T>
+  void want(T **slot, size_t count) {
+    reqs.emplace_back((void **)slot, count * sizeof(T));
+  }
+  DevBuf buf;
+  hipError_t commit() {
+    size_t total = 0;
+    for (auto &r : reqs) total += (r.second + 255) & ~(size_t)255;
+    hipError_t e = buf.alloc(total);
+    if (e != hipSuccess) return e;
+    size_t off = 0;
+    for (auto &r : reqs) {
+      *r.first = (char *)buf.p + off;
+      off += (r.second + 255) & ~(size_t)255;
+    }
+    used = total;
+    return hipSuccess;
+  }
+};
+
+struct DeviceSchedule {
+  DevBuf lvl_off, tasks;
+  uint32_t n_levels = 0, n_tasks = 0;
+};
+
+struct DeviceInstance {
+  DevBuf row_ptr, col, J, bias, g_row_ptr, g_col, chi;
+  std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by words per site G
+};
+
+}  // namespace tamc
+
+struct tamc_instance {
+  uint32_t n = 0;
+  int qubo = 0;
+  std::vector<uint32_t> row_ptr, col;  // coupling_vecs (src/ising.rs:261) as CSR
+  std::vector<float> J, bias;
+  float offset = 0.f;
+  std::vector<uint32_t> g_row_ptr, g_col;  // coupling CsMat pattern (ICM graph)
+  uint32_t g_nodes = 0;
+  std::vector<std::vector<double>> chi;
+  std::vector<uint32_t> quad_level;  // sequential-order level of every site quad
+  uint32_t n_levels = 0;
+  std::mutex mu;
+  std::unique_ptr<tamc::DeviceInstance> dev;
+};
+
+namespace tamc {
+// Upload the instance (once) and the schedule for G words per site; returns nullptr on error.
+DeviceInstance *device_instance(const tamc_instance *inst, uint32_t G, tamc_status *st);
+const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st);
+}  // namespace tamc

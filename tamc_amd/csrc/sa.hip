@@ -1,0 +1,259 @@
+// sa.hip -- simulated annealing on gfx950 (SaRunner::run / run_parallel, src/sa.rs:110-199 over
+// tamc_core::sa::simulated_annealing, tamc-core/src/sa.rs:27-48).
+//
+// Restarts are independent, so one wavefront owns 64 of them and walks the sites in the reference's
+// sequential order 0..N-1: the trajectory of every restart is exactly the sequential algorithm's.
+// The wave's spins are bit planes in LDS (one 64-bit word per site, bit = restart), so a coupling
+// row is fetched once per 64 restarts and each neighbour spin is one broadcast LDS read.  Uniforms
+// come from Philox4x32-10(ctr = {sweep, i>>2, restart, 0}, key = seed), word i&3.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "runtime_common.hpp"
+#include "tamc_math.hpp"
+
+#define TAMC_EXC_DECL static __device__ const
+#include "generated/expf_exceptions.inc"
+
+namespace tamc {
+
+struct SaTask {
+  const uint32_t *row_ptr, *col;
+  const float *J, *bias;
+  const float *betas;
+  float offset;
+  uint32_t n, K, R, rbase;  // rbase: first restart of this wave
+  uint64_t seed;
+  uint32_t *spins_out;      // [n][2] bit-plane halves of this wave's restarts
+  float *energy_out;        // [64]
+};
+
+__device__ __forceinline__ uint32_t sa_lane() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+__device__ __forceinline__ bool sa_accept(float x, uint32_t w) {
+  const uint32_t k = w >> 8;
+  const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  const float P = p * 16777216.0f;
+  const float kf = (float)k;
+  if (kf < P * (1.0f - 0x1p-12f)) return true;
+  if (kf > P * (1.0f + 0x1p-12f)) return false;
+  return accept_exact(x, k, ExpfExceptions{kExpfExcX, kExpfExcY, kExpfExcCount});
+}
+
+__global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave) {
+  extern __shared__ __align__(16) uint32_t sa_smem[];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
+  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
+  if (tix >= ntasks) return;
+  const SaTask &X = tasks[tix];
+  const uint32_t lane = sa_lane(), half = lane >> 5, sh = lane & 31u;
+  uint32_t *sp = sa_smem + (size_t)wave * words_per_wave;
+  const uint32_t n = X.n;
+  const uint32_t r = X.rbase + lane;
+  const bool valid = r < X.R;
+  const uint32_t key0 = (uint32_t)X.seed, key1 = (uint32_t)(X.seed >> 32);
+  // initial states: rand_ising_state with the production init stream
+  for (uint32_t i = 0; i < n; ++i) {
+    const U4 o = philox4x32_10(0u, i >> 2, r, kTagInit, key0, key1);
+    const uint32_t bit = valid ? (((pick(o, i) >> 31) ^ 1u) & 1u) : 0u;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(bit != 0u);
+    if (sh == 0u) sp[i * 2u + half] = half ? (uint32_t)(m >> 32) : (uint32_t)m;
+  }
+  for (uint32_t k = 0; k < X.K; ++k) {
+    const float beta = X.betas[k];
+    U4 u = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < n; ++i) {
+      if ((i & 3u) == 0u) u = philox4x32_10(k, i >> 2, r, kTagSweep, key0, key1);
+      float d = 0.0f;
+      d += X.bias[i];
+      const uint32_t e0 = X.row_ptr[i], e1 = X.row_ptr[i + 1];
+      for (uint32_t e = e0; e < e1; ++e) {
+        const float Je = X.J[e];
+        const uint32_t wj = sp[X.col[e] * 2u + half];
+        d += ((wj >> sh) & 1u) ? -Je : Je;
+      }
+      const uint32_t wi = sp[i * 2u + half];
+      const uint32_t bi = (wi >> sh) & 1u;
+      d *= bi ? 2.0f : -2.0f;
+      bool acc = d < 0.0f;
+      if (!acc) acc = sa_accept(-beta * d, pick(u, i));
+      acc = acc && valid;
+      const uint64_t m = __builtin_amdgcn_ballot_w64((bi ^ (acc ? 1u : 0u)) != 0u);
+      if (sh == 0u) sp[i * 2u + half] = half ? (uint32_t)(m >> 32) : (uint32_t)m;
+    }
+  }
+  // AnnealMinResults::apply_measurements: energy_ref of every final state (src/sa.rs:55-64)
+  float total = X.offset;
+  for (uint32_t i = 0; i < n; ++i) {
+    float h = 0.0f;
+    const float si = ((sp[i * 2u + half] >> sh) & 1u) ? -1.0f : 1.0f;
+    h += X.bias[i];
+    for (uint32_t e = X.row_ptr[i]; e < X.row_ptr[i + 1]; ++e) {
+      const float sj = ((sp[X.col[e] * 2u + half] >> sh) & 1u) ? -1.0f : 1.0f;
+      h += __fmul_rn(X.J[e], sj) / 2.0f;
+    }
+    total = __fadd_rn(total, __fmul_rn(h, si));
+  }
+  X.energy_out[lane] = total;
+  for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
+}
+
+}  // namespace tamc
+
+using namespace tamc;
+
+namespace {
+
+struct SaSolve {
+  const tamc_instance *inst;
+  tamc_sa_params P;
+  uint32_t R, waves;
+  float *d_betas;
+  uint32_t *d_spins;
+  float *d_energy;
+};
+
+tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_batch_stats *stats) {
+  Arena A;
+  uint32_t maxn = 0, ntasks = 0;
+  for (auto &S : solves) {
+    if (!S.inst || S.P.num_betas < 1 || !S.P.betas) return fail(TAMC_E_INVALID_ARG, "invalid SA parameters");
+    if (S.P.num_replicas > 1 && S.P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
+    for (uint32_t k = 1; k < S.P.num_betas; ++k)
+      if (!(S.P.betas[k] - S.P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+    // run_parallel keeps threads * (num_replicas / threads) restarts (src/sa.rs:147,161)
+    S.R = S.P.num_replicas > 1 ? S.P.threads * (S.P.num_replicas / S.P.threads) : S.P.num_replicas;
+    S.waves = (S.R + 63) / 64;
+    maxn = std::max(maxn, S.inst->n);
+    ntasks += S.waves;
+    A.want(&S.d_betas, S.P.num_betas);
+    A.want(&S.d_spins, (size_t)S.waves * S.inst->n * 2);
+    A.want(&S.d_energy, (size_t)S.waves * 64);
+  }
+  const uint32_t wpw = maxn * 2u;
+  uint32_t wpg = 4;  // waves per work group
+  while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
+  if (wpw * 4u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20480)");
+  SaTask *d_tasks;
+  A.want(&d_tasks, ntasks);
+  TAMC_HIP(A.commit());
+  std::vector<SaTask> tasks;
+  for (auto &S : solves) {
+    tamc_status st;
+    DeviceInstance *D = device_instance(S.inst, 1, &st);
+    if (!D) return st;
+    TAMC_HIP(hipMemcpy(S.d_betas, S.P.betas, S.P.num_betas * 4, hipMemcpyHostToDevice));
+    for (uint32_t w = 0; w < S.waves; ++w) {
+      SaTask t{};
+      t.row_ptr = (const uint32_t *)D->row_ptr.p;
+      t.col = (const uint32_t *)D->col.p;
+      t.J = (const float *)D->J.p;
+      t.bias = (const float *)D->bias.p;
+      t.betas = S.d_betas;
+      t.offset = S.inst->offset;
+      t.n = S.inst->n;
+      t.K = S.P.num_betas;
+      t.R = S.R;
+      t.rbase = w * 64;
+      t.seed = S.P.seed;
+      t.spins_out = S.d_spins + (size_t)w * S.inst->n * 2;
+      t.energy_out = S.d_energy + (size_t)w * 64;
+      tasks.push_back(t);
+    }
+  }
+  TAMC_HIP(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SaTask), hipMemcpyHostToDevice));
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void *)sa_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipEvent_t e0, e1;
+  TAMC_HIP(hipEventCreate(&e0));
+  TAMC_HIP(hipEventCreate(&e1));
+  const auto w0 = std::chrono::steady_clock::now();
+  TAMC_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(sa_kernel, dim3((ntasks + wpg - 1) / wpg), dim3(64 * wpg), wpg * wpw * 4u, 0, d_tasks, ntasks, wpw);
+  TAMC_HIP(hipGetLastError());
+  TAMC_HIP(hipEventRecord(e1, 0));
+  TAMC_HIP(hipEventSynchronize(e1));
+  const auto w1 = std::chrono::steady_clock::now();
+  float ms = 0;
+  TAMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  uint64_t attempts = 0;
+  for (size_t s = 0; s < solves.size(); ++s) {
+    SaSolve &S = solves[s];
+    const uint32_t n = S.inst->n;
+    std::vector<uint32_t> sp((size_t)S.waves * n * 2);
+    std::vector<float> en((size_t)S.waves * 64);
+    TAMC_HIP(hipMemcpy(sp.data(), S.d_spins, sp.size() * 4, hipMemcpyDeviceToHost));
+    TAMC_HIP(hipMemcpy(en.data(), S.d_energy, en.size() * 4, hipMemcpyDeviceToHost));
+    tamc_sa_result *R = (tamc_sa_result *)calloc(1, sizeof(tamc_sa_result));
+    R->n = n;
+    R->R = S.R;
+    R->energies = (float *)malloc(std::max<size_t>(1, S.R * 4));
+    R->final_states = (int8_t *)malloc(std::max<size_t>(1, (size_t)S.R * n));
+    for (uint32_t r = 0; r < S.R; ++r) {
+      const uint32_t w = r / 64, l = r % 64;
+      R->energies[r] = en[(size_t)w * 64 + l];
+      for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t bit = (sp[((size_t)w * n + i) * 2 + (l >> 5)] >> (l & 31)) & 1u;
+        R->final_states[(size_t)r * n + i] = bit ? -1 : 1;
+      }
+      if (r == 0 || R->energies[r] < R->min_energy) R->min_energy = R->energies[r];
+    }
+    R->attempts = (uint64_t)S.P.num_betas * n * S.R;
+    R->device_ms = ms;
+    R->timing_us = std::chrono::duration<double, std::micro>(w1 - w0).count();
+    attempts += R->attempts;
+    results[s] = R;
+  }
+  if (stats) {
+    stats->device_ms = ms;
+    stats->wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
+    stats->attempts = attempts;
+    stats->waves_per_group = wpg;
+    stats->launches = 1;
+    stats->groups = ntasks;
+  }
+  return TAMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tamc_status tamc_sa_run(const tamc_instance *inst, const tamc_sa_params *params, tamc_sa_result **out) {
+  return tamc_sa_run_batch(1, &inst, params, out, nullptr);
+}
+
+tamc_status tamc_sa_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_sa_params *params,
+                              tamc_sa_result **results, tamc_batch_stats *stats) {
+  clear_error();
+  if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
+  tamc_status st = device_init();
+  if (st) return st;
+  std::vector<SaSolve> solves(count);
+  for (uint32_t k = 0; k < count; ++k) {
+    solves[k].inst = insts[k];
+    solves[k].P = params[k];
+  }
+  return run_sa(solves, results, stats);
+}
+
+void tamc_sa_result_free(tamc_sa_result *r) {
+  if (!r) return;
+  free(r->energies);
+  free(r->final_states);
+  free(r);
+}
+
+}  // extern "C"

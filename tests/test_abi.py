@@ -1,0 +1,140 @@
+"""CPU checks of the C ABI boundary: the library loads, exports every function include/tamc_gpu.h
+declares, and its host-side parts (instance parsing/building, PtIcmRunner::new resolution, bincode
+writers) agree with the CPU restatement.  No kernel is launched here."""
+import ctypes as C
+import os
+import re
+import struct
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tamc_amd import _lib, api, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "tamc_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tamc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) <= set(names)
+
+
+def _write(lines):
+    f = tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False)
+    f.write("\n".join(lines) + "\n")
+    f.close()
+    return f.name
+
+
+@pytest.mark.parametrize("qubo", [False, True])
+def test_instance_file_parity_with_restatement(qubo):
+    rng = np.random.default_rng(4)
+    i, j, K, n = W.qubo_sparse(20, 0.3, rng) if qubo else W.random_regular3(40, rng, bias_sigma=0.2)
+    lines = ["%d %d %s" % (a, b, np.format_float_positional(np.float32(k), trim="0")) for a, b, k in zip(i, j, K)]
+    lines += ["# comment line", "3 4", "1 2 1e-3", "7 7 -0.5junk"]
+    path = _write(lines)
+    try:
+        g = api.BqmIsingInstance.from_instance_file(path, qubo=qubo)
+        o = O.Instance.from_file(path, qubo=qubo)
+    finally:
+        os.unlink(path)
+    row_ptr, col, J, bias = g.csr()
+    assert g.n == o.n and np.array_equal(row_ptr, o.row_ptr) and np.array_equal(col, o.col)
+    assert np.array_equal(J, o.J) and np.array_equal(bias, o.bias) and g.offset == o.offset
+    assert g.graph_nodes == o.g_nodes
+
+
+def test_missing_instance_file_is_io_error():
+    with pytest.raises(_lib.TamcError) as e:
+        api.BqmIsingInstance.from_instance_file("/nonexistent/instance.txt")
+    assert e.value.status == _lib.TAMC_E_IO
+
+
+def test_resolve_matches_restatement_and_reference_rules():
+    p = api.PtIcmParams(beta=api.BetaOptions.Geometric(0.2, 5.0, 32), lo_beta=1.0)
+    b, lo = p.resolve()
+    assert np.array_equal(b, O.beta_geometric(0.2, 5.0, 32)) and lo == 16
+    p = api.PtIcmParams(beta=api.BetaOptions.Geometric(0.1, 5.0, 48), lo_beta=1.0)
+    assert p.resolve()[1] == int(np.argmax(O.beta_geometric(0.1, 5.0, 48) >= 1.0))
+    # lo_num_beta takes priority and is clamped to the ladder
+    assert api.PtIcmParams(beta=api.BetaOptions.Geometric(0.2, 5.0, 8), lo_beta=1.0, lo_num_beta=3).resolve()[1] == 5
+    assert api.PtIcmParams(beta=api.BetaOptions.Geometric(0.2, 5.0, 8), lo_num_beta=100).resolve()[1] == 0
+    # lo_beta beyond the ladder -> largest beta
+    assert api.PtIcmParams(beta=api.BetaOptions.Geometric(0.2, 5.0, 8), lo_beta=9.0).resolve()[1] == 7
+    # icm without lo_beta/lo_num_beta and a decreasing ladder are errors (reference panics)
+    with pytest.raises(_lib.TamcError):
+        api.PtIcmParams(beta=api.BetaOptions.Geometric(0.2, 5.0, 8)).resolve()
+    with pytest.raises(_lib.TamcError):
+        api.PtIcmParams(beta=api.BetaOptions.Arr([1.0, 0.5]), icm=False).resolve()
+    b, lo = api.PtIcmParams(beta=api.BetaOptions.Arr([0.5, 0.5, 2.0]), icm=False).resolve()
+    assert list(b) == [0.5, 0.5, 2.0] and lo == 0
+
+
+def _fake_result(B=3, C=2, n=10, nm=2, nchi=1, slots=1, sev=2):
+    np_ = C // 2
+    keep = {}
+    R = _lib.PtResult()
+    R.n, R.B, R.C, R.npairs, R.n_meas, R.nchi = n, B, C, np_, nm, nchi
+    R.n_samp_slots, R.n_samp_events, R.nbytes = slots, sev, (n + 7) // 8
+
+    def put(name, arr, ctype):
+        a = np.ascontiguousarray(arr)
+        keep[name] = a
+        setattr(R, name, a.ctypes.data_as(ctype))
+        return a
+
+    e = put("e", np.arange(B * nm * C, dtype=np.float32) * 0.5, _lib._f32p)
+    q = put("q", np.arange(B * nm * np_, dtype=np.int32) - 3, _lib._i32p)
+    chi = put("suscept", np.arange(B * nchi * nm * np_, dtype=np.float32) + 0.25, _lib._f32p)
+    smp = put("samples", np.arange(slots * sev * C * R.nbytes, dtype=np.uint8), _lib._u8p)
+    return R, keep, (e, q, chi, smp)
+
+
+def test_thermal_samples_bincode_layout():
+    R, keep, (e, q, chi, smp) = _fake_result()
+    betas = np.array([0.5, 1.0, 2.0], np.float32)
+    path = tempfile.mktemp(suffix=".bin")
+    _lib.check(_lib.lib().tamc_write_thermal_samples(C.byref(R), betas.ctypes.data_as(_lib._f32p), 2, path.encode()))
+    data = open(path, "rb").read()
+    os.unlink(path)
+    # independent decoder of the bincode-1 layout of PtIcmThermalSamples (src/pt.rs:116-125)
+    off = 0
+
+    def take(fmt):
+        nonlocal off
+        v = struct.unpack_from("<" + fmt, data, off)
+        off += struct.calcsize("<" + fmt)
+        return v if len(v) > 1 else v[0]
+
+    assert take("B") == 2 and take("Q") == 10
+    nb = take("Q")
+    assert list(take("%df" % nb)) == list(betas)
+    ns = take("Q")
+    samples = []
+    for _ in range(ns):
+        m = take("Q")
+        samples.append([bytes(take("%dB" % take("Q"))) for _ in range(m)])
+    assert len(samples) == 1 and len(samples[0]) == 4 and samples[0][1] == bytes(smp[2:4])
+    assert take("Q") == 3
+    ee = [list(take("%df" % take("Q"))) for _ in range(3)]
+    assert ee == [list(e[k * 4:(k + 1) * 4]) for k in range(3)]
+    assert take("Q") == 3
+    qq = [take("%di" % take("Q")) for _ in range(3)]
+    assert [list(np.atleast_1d(x)) for x in qq] == [list(q[k * 2:(k + 1) * 2]) for k in range(3)]
+    assert take("Q") == 3
+    for b in range(3):
+        assert take("Q") == 1
+        assert list(np.atleast_1d(take("%df" % take("Q")))) == list(chi[b * 2:(b + 1) * 2])
+    assert off == len(data)

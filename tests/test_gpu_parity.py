@@ -1,0 +1,169 @@
+"""GPU parity: the HIP path through the C ABI against the CPU restatement (oracle/) on the same inputs.
+
+Integer/byte/index outputs are compared bit-exactly.  For +-J and QUBO couplings every energy is
+exact in fp32, so energies are compared bit-exactly too.  For Gaussian couplings the production
+kernel sums tracked energies in its own (deterministic) order, so energies carry a tolerance of
+2e-4 * (1 + |E|); spins, decisions and every integer output stay exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tamc_amd import api
+from tests.helpers import make_pair, random_states
+
+pytestmark = pytest.mark.gpu
+
+EXACT_KEYS = ["q", "samples", "gs_steps", "gs_states", "acceptance_counts", "final_states", "final_ids",
+              "num_acceptances", "round_trips", "tags", "diffusion_hist"]
+
+
+def assert_same(g, o, exact_energy=True, tol=2e-4):
+    assert g["n_meas"] == o["n_meas"]
+    for k in EXACT_KEYS:
+        assert g[k].shape == o[k].shape, k
+        assert np.array_equal(g[k], o[k]), k
+    assert g["icm_moves"] == o["icm_moves"]
+    for k in ["e", "gs_energies", "final_energies", "suscept"]:
+        assert g[k].shape == o[k].shape, k
+        if exact_energy:
+            assert np.array_equal(g[k], o[k]), k
+        else:
+            np.testing.assert_allclose(g[k], o[k], rtol=tol, atol=tol, err_msg=k)
+
+
+def test_energy_delta_overlap_pack(gpu):
+    rng = np.random.default_rng(1)
+    for kind, kw in [("2d_pm", {}), ("3d_gauss", {}), ("rr3", dict(n=50)), ("qubo", dict(n=37)),
+                     ("2d_gauss", dict(L=5))]:
+        oi, gi, _ = make_pair(kind, seed=3, **kw)
+        S = random_states(rng, 17, oi.n)
+        ge = gi.energy(S)
+        oe = np.array([oi.energy(s) for s in S], np.float32)
+        assert np.array_equal(ge, oe), kind
+        gd = gi.delta_energy(S)
+        od = np.array([[oi.delta_energy(s, i) for i in range(oi.n)] for s in S], np.float32)
+        assert np.array_equal(gd, od), kind
+    S1, S2 = random_states(rng, 9, 77), random_states(rng, 9, 77)
+    assert np.array_equal(api.overlap(S1, S2), [O.lib.tamco_overlap(O._p(a, O._i8p), O._p(b, O._i8p), 77)
+                                                for a, b in zip(S1, S2)])
+    assert np.array_equal(api.pack_bytes(S1), np.array([O.pack_bytes(s) for s in S1]))
+
+
+@pytest.mark.parametrize("kind,kw", [("2d_pm", {}), ("rr3", dict(n=96)), ("3d_pm", dict(L=3)),
+                                     ("qubo", dict(n=40, density=0.1))])
+def test_houdayer_flip_sets(gpu, kind, kw):
+    oi, gi, _ = make_pair(kind, seed=7, **kw)
+    rng = np.random.default_rng(11)
+    npairs = 40  # more than one 32-pair chunk
+    S1, S2 = random_states(rng, npairs, oi.n), random_states(rng, npairs, oi.n)
+    S2[0] = S1[0]  # a pair without q<0 sites: no move
+    ks = []
+    for a, b in zip(S1, S2):
+        cnt = O.num_negative_overlap(a, b)
+        ks.append(int(rng.integers(0, cnt)) if cnt else 0)
+    g1, g2, sizes, cl = gi.houdayer(S1, S2, ks)
+    for p in range(npairs):
+        a, b = S1[p].copy(), S2[p].copy()
+        r, ocl = oi.houdayer_k(a, b, ks[p])
+        assert sizes[p] == max(r, 0)
+        assert np.array_equal(g1[p], a) and np.array_equal(g2[p], b)
+        assert np.array_equal(cl[p], ocl)
+
+
+def _run_both(kind, kw, B=8, C=2, T=60, lo=4, threads=1, seed=5, icm=True, sample=4, sstates=4, limiting=0,
+              bmin=0.2, bmax=5.0):
+    oi, gi, _ = make_pair(kind, seed=seed, **kw)
+    params = api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(bmin, bmax, B),
+                             lo_num_beta=B - lo, icm=icm, num_replica_chains=C, threads=threads, sample=sample,
+                             sample_states=sstates, sample_limiting=limiting, seed=seed * 1000 + 17)
+    runner = api.PtIcmRunner(gi, params)
+    g = runner.run()
+    o = O.pt_run(oi, runner.betas, T, num_chains=C, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=icm,
+                 threaded=threads > 1, sample=sample, sample_states=sstates, sample_limiting=limiting,
+                 rng_mode=O.RNG_PHILOX, seed=params.seed)
+    return g, o
+
+
+@pytest.mark.parametrize("kind,kw,C,threads,limiting", [
+    ("2d_pm", {}, 2, 1, 0), ("2d_pm", {}, 2, 4, 1), ("2d_ferro", dict(L=6), 4, 1, 2), ("3d_pm", dict(L=3), 3, 1, 0),
+    ("rr3_pm", dict(n=64), 2, 1, 0), ("qubo", dict(n=24, density=0.2), 2, 1, 0), ("2d_pm", dict(L=5), 1, 1, 0)])
+def test_pt_production_exact(gpu, kind, kw, C, threads, limiting):
+    g, o = _run_both(kind, kw, C=C, threads=threads, limiting=limiting)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=True)
+
+
+@pytest.mark.parametrize("kind,kw", [("rr3", dict(n=120)), ("3d_gauss", dict(L=4)), ("2d_gauss", dict(L=6))])
+def test_pt_production_gaussian(gpu, kind, kw):
+    g, o = _run_both(kind, kw, T=30, B=6, lo=2)
+    assert_same(g, o, exact_energy=False)
+
+
+def test_pt_large_ladder_two_words(gpu):
+    # 48 betas x 2 chains = 96 replicas: two bit-plane words per site
+    g, o = _run_both("3d_pm", dict(L=3), B=48, lo=28, T=24, sample=3, sstates=5)
+    assert_same(g, o, exact_energy=True)
+
+
+def _streams(count, length, seed):
+    x = O.Xoshiro(seed)
+    out = []
+    for _ in range(count):
+        out.append(x.clone().words(length))
+        x.jump()
+    return out
+
+
+@pytest.mark.parametrize("kind,kw,threads", [("2d_pm", {}, 2), ("rr3", dict(n=40), 2), ("rr3", dict(n=40), 1),
+                                             ("qubo", dict(n=20, density=0.3), 2)])
+def test_lockstep_bit_exact(gpu, kind, kw, threads):
+    oi, gi, _ = make_pair(kind, seed=9, **kw)
+    B, C, T = 6, 2, 12
+    params = api.PtIcmParams(num_sweeps=T, warmup_fraction=0.25, beta=api.BetaOptions.Geometric(0.3, 3.0, B),
+                             lo_num_beta=3, icm=True, num_replica_chains=C, threads=threads, sample=2,
+                             sample_states=3, sample_limiting=0)
+    runner = api.PtIcmRunner(gi, params)
+    rng = np.random.default_rng(4)
+    init = random_states(rng, C * B, oi.n).reshape(C, B, oi.n)
+    ns = C * B if threads > 1 else 1
+    per = T * (oi.n + 8) * (2 if threads > 1 else C * B + 2)
+    streams = _streams(ns, per, 99)
+    g = runner.run_lockstep(init, streams)
+    o = O.pt_run(oi, runner.betas, T, num_chains=C, warmup_fraction=0.25, lo_beta_idx=runner.lo_beta_idx, icm=True,
+                 threaded=threads > 1, sample=2, sample_states=3, sample_limiting=0, rng_mode=O.RNG_FEED,
+                 streams=streams, init_states=init)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=True)
+
+
+@pytest.mark.parametrize("kind,kw,R,threads", [("qubo", dict(n=48, density=0.15), 100, 4), ("2d_gauss", {}, 64, 1),
+                                               ("rr3", dict(n=70), 1, 1)])
+def test_sa_bit_exact(gpu, kind, kw, R, threads):
+    oi, gi, _ = make_pair(kind, seed=2, **kw)
+    betas = np.linspace(0.1, 4.0, 40).astype(np.float32)
+    params = api.SaParams(beta=api.BetaOptions.Arr(betas), num_replicas=R, threads=threads, seed=77)
+    g = api.SaRunner(gi, params).run()
+    o = O.sa_run(oi, betas, R, threads=threads, rng_mode=O.RNG_PHILOX, seed=77)
+    assert g["R"] == o["R"]
+    assert np.array_equal(g["final_states"], o["final_states"])
+    assert np.array_equal(g["energies"], o["energies"])
+    assert g["min_energy"] == o["min_energy"]
+
+
+def test_batch_equals_single_and_plain(gpu):
+    insts, params = [], []
+    for k, n in enumerate([40, 100, 64, 200]):
+        _, gi, _ = make_pair("rr3", seed=20 + k, n=n)
+        insts.append(gi)
+        params.append(api.PtIcmParams(num_sweeps=40, beta=api.BetaOptions.Geometric(0.2, 5.0, 8), lo_num_beta=4,
+                                      sample=4, sample_states=8, seed=k))
+    batch, st = api.run_pt_batch(insts, params)
+    plain, _ = api.run_pt_batch(insts, params, plain=True)
+    chunked, _ = api.run_pt_batch(insts, params, sweeps_per_launch=7, use_graph=True)
+    for k in range(len(insts)):
+        single = api.PtIcmRunner(insts[k], params[k]).run()
+        for other in (plain[k], chunked[k], single):
+            for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
+                assert np.array_equal(batch[k][key], other[key]), key
+    assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
