@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: spin-flip attempts/s of the batched persistent PT-ICM kernel on one MI355X.
+
+Workload (the batched PT-ICM configuration): 256 random 3-regular graphs, N ~ U{200..4000} even,
+J ~ N(0,1) and h ~ N(0, 0.1) as float32; 32 geometric betas 0.2-5.0, 2 replica chains, 4000 sweeps,
+ICM on slots with beta >= 1.0, measurements every 32 sweeps after a 0.5 warm-up.  A "step" is one
+complete batched solve of all 256 instances (T*N*B*C Metropolis attempts summed over instances).
+Instance data is uploaded (and cached on the device) during warm-up, so the timed steps start with
+inputs resident in HBM.
+
+Reported beside it: the plain path (one instance per kernel launch, run one after another) on a
+sample of instances, and the CPU restatement of the reference on the host cores (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ATTEMPT_D3 = 9 * 3 + 9  # 4 B col + 4 B J + 1 B spin per edge; 4 B row_ptr + 4 B bias + 1 B spin per site
+
+
+def build_workload(count, seed, T):
+    from tamc_amd import api, workloads
+    insts, lines = [], []
+    for k, (i, j, K, n) in enumerate(workloads.config3_instances(count=count, seed=seed)):
+        insts.append(api.BqmIsingInstance.from_lines(i, j, K))
+        lines.append((i, j, K))
+    params = [api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 32),
+                              lo_beta=1.0, icm=True, num_replica_chains=2, threads=1, sample=32, sample_states=32,
+                              sample_limiting=2, seed=1000 + k) for k in range(count)]
+    return insts, lines, params
+
+
+def cpu_baseline(lines, params, sweeps, threads):
+    """The CPU restatement (reference algorithm: row gather per attempt, Xoshiro256++, serial
+    pt_loop) on a bounded sample: every instance for `sweeps` sweeps, instances spread over threads."""
+    import oracle as O
+    betas = O.beta_geometric(0.2, 5.0, 32)
+    lo = int(np.argmax(betas >= 1.0))
+    insts = [O.Instance.from_lines(i, j, K) for (i, j, K) in lines]
+
+    def one(k):
+        r = O.pt_run(insts[k], betas, sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_idx=lo, icm=True,
+                     sample=32, sample_states=32, sample_limiting=2, rng_mode=O.RNG_XOSHIRO, seed=k)
+        return r["attempts"]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        attempts = sum(ex.map(one, range(len(insts))))
+    dt = time.perf_counter() - t0
+    return attempts / dt, attempts, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--sweeps", type=int, default=4000)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--cpu-sweeps", type=int, default=100)
+    ap.add_argument("--cpu-instances", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--sweeps-per-launch", type=int, default=0)
+    ap.add_argument("--graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (independent instances would shard across GPUs with "
+              "no communication); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    from tamc_amd import api
+    from tamc_amd._lib import check, lib
+    check(lib().tamc_init())
+    insts, lines, params = build_workload(args.instances, 103, args.sweeps)
+    sum_n = sum(x.n for x in insts)
+    attempts_per_step = sum(p.num_sweeps * x.n * 32 * 2 for p, x in zip(params, insts))
+
+    def step():
+        return api.run_pt_batch(insts, params, waves_per_group=args.waves, sweeps_per_launch=args.sweeps_per_launch,
+                                use_graph=args.graph)
+
+    for _ in range(args.warmup):
+        step()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, st = step()
+        dev_ms.append(st["device_ms"])
+        assert st["attempts"] == attempts_per_step
+    wall = time.perf_counter() - t0
+    value = attempts_per_step * args.steps / wall
+    kernel_rate = attempts_per_step / (np.mean(dev_ms) / 1e3)
+    ms_per_step = wall / args.steps * 1e3
+
+    # plain path: one instance per launch, sequentially, on a sample of instances
+    k = min(args.plain_sample, len(insts))
+    _, pst = api.run_pt_batch(insts[:1], params[:1], plain=True)  # warm
+    t1 = time.perf_counter()
+    _, pst = api.run_pt_batch(insts[:k], params[:k], plain=True)
+    plain_wall = time.perf_counter() - t1
+    plain_rate = pst["attempts"] / plain_wall
+
+    achieved = kernel_rate * BYTES_PER_ATTEMPT_D3 / 1e9
+    out = {
+        "metric": "spin-flip attempts/s per GPU (batched PT-ICM)",
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched PT-ICM: %d random 3-regular graphs N~U{200..4000} (sum N=%d), J~N(0,1), "
+                        "h~N(0,0.1), 32 geometric betas 0.2-5.0, 2 chains, %d sweeps, ICM beta>=1.0"
+                        % (len(insts), sum_n, args.sweeps),
+            "instances": len(insts), "sum_n": sum_n, "betas": 32, "chains": 2, "sweeps": args.sweeps,
+            "waves_per_group": st["waves_per_group"], "sweeps_per_launch": st["sweeps_per_launch"],
+            "launches": st["launches"],
+        },
+        "kernel_attempts_per_s": kernel_rate,
+        "plain_gpu": {"value": plain_rate, "unit": "spin-flip attempts/s", "instances": k,
+                      "note": "one instance per kernel launch, launched one after another"},
+        "batched_vs_plain": value / plain_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_attempt": BYTES_PER_ATTEMPT_D3},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, att, dt = cpu_baseline(lines[:args.cpu_instances], params[:args.cpu_instances], args.cpu_sweeps,
+                                     threads)
+        out["cpu_baseline"] = {"value": rate, "unit": "spin-flip attempts/s", "cores": threads, "kind": "port",
+                               "sample": "%d instances x %d sweeps of the same workload (%.3g attempts, %.1f s)"
+                                         % (min(args.cpu_instances, len(lines)), args.cpu_sweeps, att, dt)}
+        out["batched_vs_cpu"] = value / rate
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -166,6 +166,8 @@ typedef struct {
   double wall_ms;
   uint64_t attempts;
   uint32_t waves_per_group, sweeps_per_launch, launches, groups;
+  double phase_ms[6];        /* slowest work group: ICM overlap+seeds, ICM cluster growth, ICM flip,
+                                exchange, sweep, measurement time */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

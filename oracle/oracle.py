@@ -237,6 +237,12 @@ class Instance:
         return lib.tamco_suscept(self.h, k, _p(ov, _i8p))
 
 
+def overlap(a, b):
+    a = np.ascontiguousarray(a, np.int8)
+    b = np.ascontiguousarray(b, np.int8)
+    return int(lib.tamco_overlap(_p(a, _i8p), _p(b, _i8p), len(a)))
+
+
 def num_negative_overlap(s1, s2):
     return lib.tamco_num_negative_overlap(_p(s1, _i8p), _p(s2, _i8p), len(s1))
 

@@ -308,7 +308,43 @@ float sweep(const Inst &I, int8_t *s, float beta, U &&draw) {
   return de;
 }
 
-// Philox production uniforms for one replica sweep; the quad block is computed on first use.
+// Production RNG grouping of the GPU path (not a reference concept): level(i) = 1 + max level of the
+// lower-numbered sites in row i (0 if none); within a level, sites in ascending order form groups of
+// four numbered level-major.  The PT uniform for (sweep t, site i, replica rho) is word slot(i) of
+// Philox4x32-10(ctr = {t, group(i), rho, 0}).
+struct RngGroups {
+  std::vector<uint32_t> gid, slot;
+  explicit RngGroups(const Inst &I) : gid(I.n), slot(I.n) {
+    std::vector<uint32_t> lv(I.n, 0);
+    uint32_t maxl = 0;
+    for (uint32_t i = 0; i < I.n; ++i) {
+      uint32_t l = 0;
+      for (uint32_t e = I.row_ptr[i]; e < I.row_ptr[i + 1]; ++e)
+        if (I.col[e] < i) l = std::max(l, lv[I.col[e]] + 1);
+      lv[i] = l;
+      maxl = std::max(maxl, l);
+    }
+    std::vector<std::vector<uint32_t>> by(I.n ? maxl + 1 : 0);
+    for (uint32_t i = 0; i < I.n; ++i) by[lv[i]].push_back(i);
+    uint32_t base = 0;
+    for (auto &v : by) {
+      for (size_t k = 0; k < v.size(); ++k) {
+        gid[v[k]] = base + (uint32_t)(k / 4);
+        slot[v[k]] = (uint32_t)(k % 4);
+      }
+      base += (uint32_t)((v.size() + 3) / 4);
+    }
+  }
+};
+
+struct PhiloxGroupDraw {
+  uint64_t seed;
+  uint32_t t, rho;
+  const RngGroups *G;
+  float operator()(uint32_t i) const { return u24(philox_word(seed, t, G->gid[i], rho, TAG_SWEEP, G->slot[i])); }
+};
+
+// Philox uniforms keyed by site quads (SA path); the quad block is computed on first use.
 struct PhiloxSweepDraw {
   uint64_t seed;
   uint32_t t, rho;
@@ -469,8 +505,9 @@ struct PtRun {
   std::vector<float> gs_e;
   std::vector<uint64_t> gs_states;
   uint64_t icm_moves = 0, attempts = 0;
+  RngGroups groups;
 
-  PtRun(const Inst &I_, const tamco_pt_params &P_) : I(I_), P(P_) {
+  PtRun(const Inst &I_, const tamco_pt_params &P_) : I(I_), P(P_), groups(I_) {
     n = I.n;
     B = P.num_betas;
     C = P.num_chains;
@@ -582,8 +619,7 @@ struct PtRun {
       St &x = ch.st[b];
       float de;
       if (P.rng_mode == TAMCO_RNG_PHILOX) {
-        PhiloxSweepDraw d{P.seed, t, x.id};
-        de = sweep(I, x.s.data(), beta[b], d);
+        de = sweep(I, x.s.data(), beta[b], PhiloxGroupDraw{P.seed, t, x.id, &groups});
       } else {
         WordStream &ws = streams[stream_index(c, b)];
         de = sweep(I, x.s.data(), beta[b], [&](uint32_t) { return ws.next_f32(); });

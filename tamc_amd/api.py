@@ -280,7 +280,7 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
             lib().tamc_pt_result_free(outs[k])
     st = dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
-              launches=stats.launches, groups=stats.groups)
+              launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms))
     return res, st
 
 

@@ -11,6 +11,7 @@
 #include <map>
 #include <string>
 
+#include "pt_types.hpp"
 #include "runtime_common.hpp"
 
 namespace tamc {
@@ -107,20 +108,29 @@ static void adj_put(Adj &adj, uint32_t i, uint32_t j, double K, bool verbose) {
 }
 
 static void compute_levels(tamc_instance *I) {
-  const uint32_t nq = (I->n + 3u) / 4u;
-  I->quad_level.assign(nq, 0u);
+  const uint32_t n = I->n;
+  std::vector<uint32_t> lv(n, 0u);
   uint32_t maxl = 0;
-  for (uint32_t q = 0; q < nq; ++q) {
-    uint32_t lv = 0;
-    for (uint32_t i = 4u * q; i < std::min(I->n, 4u * q + 4u); ++i)
-      for (uint32_t e = I->row_ptr[i]; e < I->row_ptr[i + 1]; ++e) {
-        const uint32_t qj = I->col[e] / 4u;
-        if (qj < q) lv = std::max(lv, I->quad_level[qj] + 1u);
-      }
-    I->quad_level[q] = lv;
-    maxl = std::max(maxl, lv);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t l = 0;
+    for (uint32_t e = I->row_ptr[i]; e < I->row_ptr[i + 1]; ++e) {
+      const uint32_t j = I->col[e];
+      if (j < i) l = std::max(l, lv[j] + 1u);
+    }
+    lv[i] = l;
+    maxl = std::max(maxl, l);
   }
-  I->n_levels = nq ? maxl + 1u : 0u;
+  I->n_levels = n ? maxl + 1u : 0u;
+  std::vector<std::vector<uint32_t>> by(I->n_levels);
+  for (uint32_t i = 0; i < n; ++i) by[lv[i]].push_back(i);
+  I->level_groups.assign(I->n_levels + 1, 0u);
+  I->group_sites.clear();
+  for (uint32_t l = 0; l < I->n_levels; ++l) {
+    for (size_t k = 0; k < by[l].size(); k += 4)
+      for (size_t s = 0; s < 4; ++s) I->group_sites.push_back(k + s < by[l].size() ? by[l][k + s] : kNone);
+    I->level_groups[l + 1] = (uint32_t)(I->group_sites.size() / 4);
+  }
+  I->n_groups = (uint32_t)(I->group_sites.size() / 4);
 }
 
 static tamc_instance *build(const Adj &adj, int qubo) {
@@ -195,19 +205,54 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
     *st = fail(TAMC_E_DEVICE, "instance upload failed");
     return nullptr;
   }
+  if (!D->ell_col.p) {
+    // fixed-stride rows (ELL) when every row has at most 8 entries: a quad's couplings then load
+    // with a few wide scalar loads; padding adds +0.0, which leaves every fp32 sum unchanged
+    uint32_t dmax = 0;
+    for (uint32_t i = 0; i < inst->n; ++i) dmax = std::max(dmax, inst->row_ptr[i + 1] - inst->row_ptr[i]);
+    const uint32_t Dl = dmax <= 4 ? 4u : (dmax <= 8 ? 8u : 0u);
+    std::vector<uint32_t> ec((size_t)inst->n * std::max(Dl, 1u));
+    std::vector<float> eJ(ec.size(), 0.0f);
+    for (uint32_t i = 0; i < inst->n && Dl; ++i)
+      for (uint32_t k = 0; k < Dl; ++k) {
+        const uint32_t e = inst->row_ptr[i] + k;
+        const bool real = e < inst->row_ptr[i + 1];
+        ec[(size_t)i * Dl + k] = real ? inst->col[e] : i;
+        eJ[(size_t)i * Dl + k] = real ? inst->J[e] : 0.0f;
+      }
+    std::vector<uint16_t> ec16(ec.size());
+    for (size_t k = 0; k < ec.size(); ++k) ec16[k] = (uint16_t)ec[k];
+    if (!up(D->ell_col, ec.data(), ec.size() * 4) || !up(D->ell_J, eJ.data(), eJ.size() * 4) ||
+        (inst->n < 65536u && !up(D->ell_col16, ec16.data(), ec16.size() * 2))) {
+      *st = fail(TAMC_E_DEVICE, "instance upload failed");
+      return nullptr;
+    }
+    D->ell_D = Dl;
+    std::vector<float> sj(inst->n, 0.0f);
+    for (uint32_t i = 0; i < inst->n; ++i)
+      for (uint32_t e = inst->row_ptr[i]; e < inst->row_ptr[i + 1]; ++e)
+        if (inst->col[e] == i) {
+          sj[i] = inst->J[e];
+          D->has_self = true;
+        }
+    if (!up(D->self_J, sj.data(), sj.size() * 4)) {
+      *st = fail(TAMC_E_DEVICE, "instance upload failed");
+      return nullptr;
+    }
+  }
   if (!D->sched.count(G)) {
     std::unique_ptr<DeviceSchedule> S(new DeviceSchedule());
-    const uint32_t nq = (uint32_t)inst->quad_level.size();
-    std::vector<uint32_t> cnt(inst->n_levels + 1, 0u);
-    for (uint32_t q = 0; q < nq; ++q) cnt[inst->quad_level[q] + 1] += G;
-    for (uint32_t l = 0; l < inst->n_levels; ++l) cnt[l + 1] += cnt[l];
-    std::vector<uint32_t> tasks(cnt.back());
-    std::vector<uint32_t> fill(cnt.begin(), cnt.end());
-    for (uint32_t q = 0; q < nq; ++q)
-      for (uint32_t g = 0; g < G; ++g) tasks[fill[inst->quad_level[q]]++] = q * G + g;
+    // tasks (group, word): level-major, word-minor
+    std::vector<uint32_t> cnt(inst->n_levels + 1, 0u), tasks;
+    for (uint32_t l = 0; l < inst->n_levels; ++l) {
+      for (uint32_t q = inst->level_groups[l]; q < inst->level_groups[l + 1]; ++q)
+        for (uint32_t g = 0; g < G; ++g) tasks.push_back(q * G + g);
+      cnt[l + 1] = (uint32_t)tasks.size();
+    }
     S->n_levels = inst->n_levels;
     S->n_tasks = (uint32_t)tasks.size();
-    if (!up(S->lvl_off, cnt.data(), cnt.size() * 4) || !up(S->tasks, tasks.data(), tasks.size() * 4)) {
+    if (!up(S->lvl_off, cnt.data(), cnt.size() * 4) || !up(S->tasks, tasks.data(), tasks.size() * 4) ||
+        !up(S->groups, inst->group_sites.data(), inst->group_sites.size() * 4)) {
       *st = fail(TAMC_E_DEVICE, "schedule upload failed");
       return nullptr;
     }

@@ -464,7 +464,7 @@ extern "C" tamc_status tamc_pt_run_lockstep(const tamc_instance *inst, const tam
   for (uint32_t k = 0; k < R; ++k) {
     perm[k] = k;
     const uint32_t b = k % B;
-    tags[k] = b == 0 ? 1 : (b == B - 1 ? 2 : 0);
+    tags[k] = b == B - 1 ? 2 : (b == 0 ? 1 : 0);  // PTState::new: the last slot's MaxBeta wins
   }
   TAMC_HIP(hipMemcpy(L.perm, perm.data(), R * 4, hipMemcpyHostToDevice));
   TAMC_HIP(hipMemcpy(L.tags, tags.data(), R, hipMemcpyHostToDevice));

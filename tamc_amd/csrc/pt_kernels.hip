@@ -8,12 +8,13 @@
 //   3. Metropolis sweep of every replica                           (src/ising.rs:225-233)
 //   4. measurements and ground-state tracking after warm-up        (src/pt.rs:542-583)
 //
-// The sweep visits site quads level by level.  A quad's level is one more than the highest level
-// of any lower-numbered quad it is coupled to, so each site sees exactly the neighbour spins it
-// would see in the reference's sequential 0..N-1 order; within a quad one lane updates its replica's
-// four sites in order.  Spin trajectories therefore equal the sequential algorithm's given the same
-// uniforms.  The uniform for (sweep t, site i, replica rho) is word i&3 of
-// Philox4x32-10(ctr = {t, i>>2, rho, 0}, key = seed): independent of tiling and of batch placement.
+// The sweep visits sites level by level: a site's level is one more than the highest level of the
+// lower-numbered sites it is coupled to, so every site sees exactly the neighbour spins it would see
+// in the reference's sequential 0..N-1 order, and spin trajectories equal the sequential
+// algorithm's given the same uniforms.  Sites of a level are uncoupled; in ascending order they
+// form groups of four, and the uniform for (sweep t, site, replica rho) is word `slot` of
+// Philox4x32-10(ctr = {t, group, rho, 0}, key = seed): a function of the instance, not of the
+// tiling or of batch placement.
 #include <hip/hip_runtime.h>
 
 #include "pt_types.hpp"
@@ -23,6 +24,14 @@
 #include "generated/expf_exceptions.inc"
 
 namespace tamc {
+
+// Read-only device data is accessed through the constant address space: wave-uniform indices then
+// become scalar (SMEM) loads, divergent ones ordinary global loads.
+#define TAMC_AS4 __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const TAMC_AS4 T *c4(const T *p) {
+  return (const TAMC_AS4 T *)p;
+}
 
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -68,7 +77,7 @@ __device__ __forceinline__ uint32_t sbit(const uint32_t *sp, uint32_t G, uint32_
 // One work group per PtGroup; lane = physical replica.
 // init_bits: nullable [n][G][2] bit planes prepared by the host from caller init states.
 __global__ void pt_init_kernel(const PtGroup *groups, const uint32_t *const *init_bits) {
-  const PtGroup &X = groups[blockIdx.x];
+  const TAMC_AS4 PtGroup &X = *c4(groups + blockIdx.x);
   const uint32_t n = X.n, G = X.G, R = X.R, B = X.B;
   const uint32_t key0 = (uint32_t)X.seed, key1 = (uint32_t)(X.seed >> 32);
   const uint32_t *ib = init_bits ? init_bits[blockIdx.x] : nullptr;
@@ -100,19 +109,21 @@ __global__ void pt_init_kernel(const PtGroup *groups, const uint32_t *const *ini
     for (uint32_t i = 0; i < n; ++i) {
       float h = 0.0f;
       const float si = sbit(X.spins, G, i, r) ? -1.0f : 1.0f;
-      h += X.bias[i];
-      for (uint32_t e = X.row_ptr[i]; e < X.row_ptr[i + 1]; ++e) {
-        const float sj = sbit(X.spins, G, X.col[e], r) ? -1.0f : 1.0f;
-        h += __fmul_rn(X.J[e], sj) / 2.0f;
+      h += c4(X.bias)[i];
+      for (uint32_t e = c4(X.row_ptr)[i]; e < c4(X.row_ptr)[i + 1]; ++e) {
+        const float sj = sbit(X.spins, G, c4(X.col)[e], r) ? -1.0f : 1.0f;
+        h += __fmul_rn(c4(X.J)[e], sj) / 2.0f;
       }
       total = __fadd_rn(total, __fmul_rn(h, si));
     }
     X.energy[r] = total;
+    X.drift[r] = 0;
   }
+  if (threadIdx.x < 6 && X.phase_ticks) X.phase_ticks[threadIdx.x] = 0ull;
   for (uint32_t k = threadIdx.x; k < X.nch * B; k += blockDim.x) {
     const uint32_t b = k % B;
     X.perm[k] = k;  // chain c slot b holds replica c*B+b
-    X.tags[k] = (b == 0) ? 1u : ((b == B - 1) ? 2u : 0u);  // MinBeta / MaxBeta / None
+    X.tags[k] = (b == B - 1) ? 2u : ((b == 0) ? 1u : 0u);  // MaxBeta / MinBeta / None (PTState::new)
     X.nacc[k] = 0;
     X.hist[2 * k] = 0;
     X.hist[2 * k + 1] = 0;
@@ -127,24 +138,124 @@ __global__ void pt_init_kernel(const PtGroup *groups, const uint32_t *const *ini
 
 // --------------------------------------------------------------------------------------------
 struct Ctx {
-  const PtGroup *X;
+  const TAMC_AS4 PtGroup *X;
   uint32_t *sp;      // LDS bit planes
   float *energy, *beta_of, *dEx;
-  long long *partial;
+  long long *partial, *drift;
   uint32_t *perm, *pair_a, *pair_b, *pair_of, *seed_blk, *seed_rank, *misc;
   uint8_t *tags;
-  uint32_t *qw, *cw, *blk;
+  uint32_t *nacc, *hist, *rt;
+  uint32_t *qw, *cw, *blk, *uw;
+  const uint16_t *nbr;  // ICM neighbour lists [n][D] (LDS copy or global)
+  long long *dEa;
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
 };
 
-// 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).
-__device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
-  const PtGroup &X = *c.X;
+// Spin bits of one site for replica indices chosen at run time.  With one 64-bit plane word per
+// site (at most 64 replicas) the word is held in a register and a bit is one 64-bit shift; wider
+// groups read the bit from LDS.  (Selecting among several registers by a run-time index makes the
+// compiler spill them to scratch memory.)
+struct SiteWords {
+  uint64_t w;
+  const uint32_t *sp;
+  uint32_t i, G;
+  __device__ __forceinline__ void load(const uint32_t *sp_, uint32_t i_, uint32_t G_) {
+    sp = sp_;
+    i = i_;
+    G = G_;
+    w = (G == 1u) ? (((uint64_t)sp[i * 2u + 1u] << 32) | sp[i * 2u]) : 0ull;
+  }
+  __device__ __forceinline__ uint32_t bit(uint32_t r) const {
+    return (G == 1u) ? (uint32_t)((w >> r) & 1ull) : sbit(sp, G, i, r);
+  }
+};
+
+// ICM step (d) over sites, lane = site.  For fixed-stride rows (D > 0) the site's couplings and its
+// neighbours' cluster words are loaded once and reused for every pair the site belongs to.
+template <int D>
+__device__ __forceinline__ void icm_flip(Ctx &c) {
+  const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t P = c.P, n = c.n, G = c.G;
+  constexpr int DD = D > 0 ? D : 1;
+  for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+    const uint32_t i = blk * 64u + c.lane;
+    const uint32_t cwi = (i < n) ? c.cw[i] : 0u;
+    if (ballot(cwi != 0u) == 0ull) continue;
+    SiteWords h{0ull, c.sp, 0u, G};
+    unsigned long long M[4] = {0, 0, 0, 0};
+    float bias_i = 0.f;
+    uint32_t nj[DD], ncw[DD];
+    float nJ[DD];
+#pragma unroll
+    for (int k = 0; k < DD; ++k) {
+      nj[k] = 0u;
+      ncw[k] = ~0u;
+      nJ[k] = 0.f;
+    }
+    if (cwi) {
+      h.load(c.sp, i, G);
+      bias_i = c4(X.bias)[i];
+      if constexpr (D > 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          nj[k] = c4(X.ell_col)[(size_t)i * D + k];
+          nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) ncw[k] = c.cw[nj[k]];
+      }
+    }
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint32_t bp = (cwi >> p) & 1u;
+      if (ballot(bp != 0u) == 0ull) continue;
+      const uint32_t a = c.pair_a[p], b = c.pair_b[p];
+      if (bp) {
+        float hh = bias_i;
+        if constexpr (D > 0) {
+          uint32_t sj[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) sj[k] = sbit(c.sp, G, nj[k], a);
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            if (!((ncw[k] >> p) & 1u)) hh += sj[k] ? -nJ[k] : nJ[k];
+        } else {
+          const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
+          for (uint32_t e = e0; e < e1; ++e) {
+            const uint32_t j = c4(X.col)[e];
+            if (!((c.cw[j] >> p) & 1u)) {
+              const float Je = c4(X.J)[e];
+              hh += sbit(c.sp, G, j, a) ? -Je : Je;
+            }
+          }
+        }
+        const float term = h.bit(a) ? 2.0f * hh : -2.0f * hh;
+        atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)to_fix(term));
+#pragma unroll
+        for (uint32_t g = 0; g < 4u; ++g) {
+          if ((a >> 6) == g) M[g] |= 1ull << (a & 63u);
+          if ((b >> 6) == g) M[g] |= 1ull << (b & 63u);
+        }
+      }
+    }
+#pragma unroll
+    for (uint32_t g = 0; g < 4u; ++g)
+      if (g < G && M[g]) {
+        c.sp[(i * G + g) * 2u] ^= (uint32_t)M[g];
+        c.sp[(i * G + g) * 2u + 1u] ^= (uint32_t)(M[g] >> 32);
+      }
+  }
+}
+
+// 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
+// the two chains; bit p of the per-site words qw/cw belongs to pair p.
+__device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long *ticks, unsigned long long &tk) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t P = c.P, n = c.n, G = c.G, D = c.nbr ? X.ell_D : 0u;
   if (c.tid < P) {
     const uint32_t b = c.lo + c.tid;
     c.pair_a[c.tid] = c.perm[b];
     c.pair_b[c.tid] = c.perm[c.B + b];
+    c.dEa[c.tid] = 0;
   }
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) c.pair_of[r] = kNone;
   __syncthreads();
@@ -152,18 +263,26 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     c.pair_of[c.pair_a[c.tid]] = c.tid;
     c.pair_of[c.pair_b[c.tid]] = c.tid;
   }
-  // (a) overlap words: bit p of qw[i] <=> replicas of pair p disagree at site i; block counts
+  // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; per-block counts
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
     const uint32_t i = blk * 64u + c.lane;
     const bool valid = i < n;
-    const uint32_t ii = valid ? i : 0u;
+    SiteWords h{0ull, c.sp, 0u, G};
+    if (valid) h.load(c.sp, i, G);
     uint32_t q = 0;
-    for (uint32_t p = 0; p < P; ++p) {
-      const uint32_t a = c.pair_a[p], b = c.pair_b[p];
-      const uint32_t qb = (valid ? (sbit(c.sp, G, ii, a) ^ sbit(c.sp, G, ii, b)) : 0u);
-      q |= qb << p;
-      const uint32_t cnt = popc64(ballot(qb != 0u));
-      if (c.lane == 0) c.blk[blk * P + p] = cnt;
+    for (uint32_t p0 = 0; p0 < P; p0 += 4u) {
+      const uint4 A = reinterpret_cast<const uint4 *>(c.pair_a)[p0 >> 2];
+      const uint4 Bp = reinterpret_cast<const uint4 *>(c.pair_b)[p0 >> 2];
+      const uint32_t av[4] = {A.x, A.y, A.z, A.w}, bv[4] = {Bp.x, Bp.y, Bp.z, Bp.w};
+#pragma unroll
+      for (uint32_t s = 0; s < 4u; ++s) {
+        const uint32_t p = p0 + s;
+        if (p >= P) break;
+        const uint32_t qb = valid ? (h.bit(av[s]) ^ h.bit(bv[s])) : 0u;
+        q |= qb << p;
+        const uint32_t cnt = popc64(ballot(qb != 0u));
+        if (c.lane == 0) c.blk[blk * P + p] = cnt;
+      }
     }
     if (valid) {
       c.qw[i] = q;
@@ -175,12 +294,13 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   if (c.wave == 0 && c.lane < P) {
     const uint32_t p = c.lane;
     uint32_t total = 0;
+#pragma unroll 8
     for (uint32_t blk = 0; blk < c.nblk; ++blk) total += c.blk[blk * P + p];
     uint32_t sb = kNone, sr = 0;
     if (total) {
       uint32_t k;
       if (X.icm_k) {
-        k = X.icm_k[p];
+        k = c4(X.icm_k)[p];
         if (k >= total) k = total - 1u;
       } else {
         const U4 v = philox4x32_10(t, c.lo + p, X.pair_global, kTagIcm, c.key0, c.key1);
@@ -201,32 +321,35 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     c.seed_rank[p] = sr;
   }
   __syncthreads();
-  if (c.wave == 0) {
-    uint32_t moves = 0;
-    for (uint32_t p = 0; p < P; ++p) {
-      const uint32_t sb = c.seed_blk[p];
-      if (sb == kNone) continue;
-      ++moves;
-      const uint32_t i = sb * 64u + c.lane;
-      const uint32_t qb = (i < n) ? ((c.qw[i] >> p) & 1u) : 0u;
-      const uint64_t m = ballot(qb != 0u);
-      if (qb && mbcnt64(m) == c.seed_rank[p]) c.cw[i] |= (1u << p);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    }
-    if (c.lane == 0 && moves) atomicAdd(X.icm_moves, (unsigned long long)moves);
+  for (uint32_t p = c.wave; p < P; p += c.W) {
+    const uint32_t sb = c.seed_blk[p];
+    if (sb == kNone) continue;
+    const uint32_t i = sb * 64u + c.lane;
+    const uint32_t qb = (i < n) ? ((c.qw[i] >> p) & 1u) : 0u;
+    const uint64_t m = ballot(qb != 0u);
+    if (qb && mbcnt64(m) == c.seed_rank[p]) atomicOr(&c.cw[i], 1u << p);
+    if (c.lane == 0) atomicAdd(X.icm_moves, 1ull);
   }
-  __syncthreads();
-  // (c) cluster labels: grow every pair's cluster inside its q<0 sites until nothing changes
+  {
+    const unsigned long long now = wall_clock64();
+    ticks[0] += now - tk;
+    tk = now;
+  }
+  // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
+  //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing.
   for (;;) {
+    __syncthreads();
     int changed = 0;
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
       const uint32_t q = c.qw[i];
       if (!q) continue;
       const uint32_t cur = c.cw[i];
       uint32_t nb = 0;
-      if (i < X.g_nodes) {
-        const uint32_t e0 = X.g_row_ptr[i], e1 = X.g_row_ptr[i + 1];
-        for (uint32_t e = e0; e < e1; ++e) nb |= c.cw[X.g_col[e]];
+      if (D) {
+        for (uint32_t k = 0; k < D; ++k) nb |= c.cw[c.nbr[i * D + k]];
+      } else if (i < X.g_nodes) {
+        const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
+        for (uint32_t e = e0; e < e1; ++e) nb |= c.cw[c4(X.g_col)[e]];
       }
       const uint32_t nc = cur | (q & nb);
       if (nc != cur) {
@@ -236,168 +359,187 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     }
     if (!__syncthreads_or(changed)) break;
   }
-  // (d) swap the cluster spins of both replicas (= flip both) and update their energies
-  //     incrementally: dE_r = sum_{i in cl} -2 s_i (h_i + sum_{j notin cl} J_ij s_j)
-  long long acc[4] = {0, 0, 0, 0};
-  for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
-    const uint32_t i0 = blk * 64u;
-    const uint32_t il = i0 + c.lane;
-    uint32_t cwl = 0u;
-    if (il < n) cwl = c.cw[il];
-    uint64_t m = ballot(cwl != 0u);
-    while (m) {
-      const uint32_t off = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      const uint32_t i = uni(i0 + off);
-      const uint32_t cl = c.cw[i];
-#pragma unroll
-      for (uint32_t g = 0; g < 4u; ++g) {
-        if (g >= G) break;
-        const uint32_t r = g * 64u + c.lane;
-        const uint32_t pr = (r < c.R) ? c.pair_of[r] : kNone;
-        const bool in = (pr != kNone) && ((cl >> pr) & 1u);
-        if (ballot(in) == 0ull) continue;
-        const uint32_t w = c.sp[(i * G + g) * 2u + (c.lane >> 5)];
-        const uint32_t bi = (w >> (c.lane & 31u)) & 1u;
-        if (in) {
-          float h = X.bias[i];
-          const uint32_t e0 = X.row_ptr[i], e1 = X.row_ptr[i + 1];
-          for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t j = X.col[e];
-            if (!((c.cw[j] >> pr) & 1u)) {
-              const float Je = X.J[e];
-              h += sbit(c.sp, G, j, r) ? -Je : Je;
-            }
-          }
-          const float term = bi ? 2.0f * h : -2.0f * h;
-          acc[g] += to_fix(term);
-        }
-        const uint64_t nw = ballot((bi ^ (in ? 1u : 0u)) != 0u);
-        if ((c.lane & 31u) == 0u) c.sp[(i * G + g) * 2u + (c.lane >> 5)] = (c.lane >> 5) ? (uint32_t)(nw >> 32) : (uint32_t)nw;
-      }
-    }
+  {
+    const unsigned long long now = wall_clock64();
+    ticks[1] += now - tk;
+    tk = now;
   }
-#pragma unroll
-  for (uint32_t g = 0; g < 4u; ++g)
-    if (g < G) c.partial[(c.wave * G + g) * 64u + c.lane] = acc[g];
+  // (d) swap the cluster spins of both replicas (= flip both), lane = site.  The first replica's
+  //     energy change dE_a = sum_{i in cl} -2 s_a(i) (h_i + sum_{j notin cl} J_ij s_a(j)) is summed in
+  //     fixed point; the second replica's is -dE_a (the move conserves E_a + E_b).
+  if (X.ell_D == 4u) icm_flip<4>(c);
+  else if (X.ell_D == 8u) icm_flip<8>(c);
+  else icm_flip<0>(c);
   __syncthreads();
-  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
-    if (c.pair_of[r] == kNone) continue;
-    const uint32_t g = r >> 6, l = r & 63u;
-    long long s = 0;
-    for (uint32_t w = 0; w < c.W; ++w) s += c.partial[(w * G + g) * 64u + l];
-    c.energy[r] = add_fix(c.energy[r], s);
+  // moved pairs: energies become energy_ref of the new states, as the reference recomputes them
+  if (c.tid < P && c.seed_blk[c.tid] != kNone) {
+    const uint32_t a = c.pair_a[c.tid], b = c.pair_b[c.tid];
+    const long long d = c.dEa[c.tid];
+    c.energy[a] = add_fix(c.energy[a], d - c.drift[a]);
+    c.energy[b] = add_fix(c.energy[b], -d - c.drift[b]);
+    c.drift[a] = 0;
+    c.drift[b] = 0;
   }
   __syncthreads();
 }
 
-// 2. replica exchange and round trips, one lane per chain of the group
+// 2. replica exchange (tamc-core/src/pt.rs:126-144; threaded variant tamc-core/src/parallel/pt.rs:202-215)
+//    and round trips (tamc-core/src/pt.rs:54-84).  Energy differences and uniforms are prepared in
+//    parallel; one lane per chain then walks the ladder carrying only registers.
 __device__ __forceinline__ void exchange_phase(Ctx &c, uint32_t t) {
-  const PtGroup &X = *c.X;
-  const uint32_t B = c.B;
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t B = c.B, B1 = B - 1u;
+  for (uint32_t k = c.tid; k < c.nch * B1; k += c.nthreads) {
+    const uint32_t ch = k / B1, j = k - ch * B1;
+    const uint32_t *perm = c.perm + ch * B;
+    c.dEx[ch * B + j] = c.energy[perm[j + 1]] - c.energy[perm[j]];
+    const U4 v = philox4x32_10(t, j >> 2, X.chain0 + ch, kTagExchange, c.key0, c.key1);
+    c.uw[ch * B + j] = pick(v, j);
+  }
+  __syncthreads();
   if (c.wave == 0 && c.lane < c.nch) {
     const uint32_t ch = c.lane;
-    const uint32_t cg = X.chain0 + ch;
     uint32_t *perm = c.perm + ch * B;
     uint8_t *tags = c.tags + ch * B;
-    float *dE = c.dEx + ch * B;
-    for (uint32_t j = 0; j + 1 < B; ++j) dE[j] = c.energy[perm[j + 1]] - c.energy[perm[j]];
+    const float *dE = c.dEx + ch * B;
+    const uint32_t *uw = c.uw + ch * B;
     const bool threaded = (c.flags & kFlagThreaded) != 0u;
-    U4 v = {0, 0, 0, 0};
-    uint32_t vblk = kNone;
-    for (uint32_t j = 0; j + 1 < B; ++j) {
-      const float dbeta = X.betas[j + 1] - X.betas[j];
-      const float dlt = dbeta * dE[j];
+    uint32_t cur = perm[0];
+    uint8_t curtag = tags[0];
+    bool prev_ok = false;
+    float prev = 0.0f;
+    for (uint32_t j = 0; j < B1; ++j) {
+      // serial variant: after a swap at j-1, dE[j] += dE[j-1] (the value used at j-1)
+      const float eff = (!threaded && prev_ok) ? dE[j] + prev : dE[j];
+      const float dbeta = c4(X.betas)[j + 1] - c4(X.betas)[j];
+      const float dlt = dbeta * eff;
       bool ok = dlt >= 0.0f;
-      if (!ok) {
-        if ((j >> 2) != vblk) {
-          vblk = j >> 2;
-          v = philox4x32_10(t, vblk, cg, kTagExchange, c.key0, c.key1);
-        }
-        ok = accept_u(dlt, pick(v, j));
-      }
+      if (!ok) ok = accept_u(dlt, uw[j]);
+      const uint32_t nxt = perm[j + 1];
+      const uint8_t nxttag = tags[j + 1];
       if (ok) {
-        const uint32_t tp = perm[j];
-        perm[j] = perm[j + 1];
-        perm[j + 1] = tp;
-        const uint8_t tt = tags[j];
-        tags[j] = tags[j + 1];
-        tags[j + 1] = tt;
-        X.nacc[ch * B + j] += 1u;
-        if (!threaded) {
-          if (j + 2 < B) dE[j + 1] = dE[j + 1] + dE[j];
-          dE[j] = -dE[j];
-        }
+        perm[j] = nxt;
+        tags[j] = nxttag;
+        c.nacc[ch * B + j] += 1u;
+      } else {
+        perm[j] = cur;
+        tags[j] = curtag;
+        cur = nxt;
+        curtag = nxttag;
       }
+      prev_ok = ok;
+      prev = eff;
     }
-    // update_round_trips (tamc-core/src/pt.rs:54-84); tags: 0 None 1 Min 2 Max 3 MinRefl 4 MaxRefl
+    perm[B1] = cur;
+    tags[B1] = curtag;
+    // update_round_trips; tags: 0 None 1 Min 2 Max 3 MinRefl 4 MaxRefl
     uint8_t lo = tags[0];
     if (lo == 0) lo = 1;
     else if (lo == 4) {
-      X.rt[ch] += 1u;
+      c.rt[ch] += 1u;
       lo = 1;
     } else if (lo == 2) lo = 3;
     tags[0] = lo;
-    uint8_t hi = tags[B - 1];
+    uint8_t hi = tags[B1];
     if (hi == 0) hi = 2;
     else if (hi == 3) {
-      X.rt[ch] += 1u;
+      c.rt[ch] += 1u;
       hi = 2;
     } else if (hi == 1) hi = 4;
-    tags[B - 1] = hi;
-    for (uint32_t b = 0; b < B; ++b) {
-      const uint8_t tg = tags[b];
-      if (tg == 1 || tg == 3) X.hist[(ch * B + b) * 2u] += 1u;
-      else if (tg == 2 || tg == 4) X.hist[(ch * B + b) * 2u + 1u] += 1u;
-    }
+    tags[B1] = hi;
   }
   __syncthreads();
-  for (uint32_t k = c.tid; k < c.nch * B; k += c.nthreads) c.beta_of[c.perm[k]] = X.betas[k % B];
+  for (uint32_t k = c.tid; k < c.nch * B; k += c.nthreads) {
+    const uint8_t tg = c.tags[k];
+    if (tg == 1 || tg == 3) c.hist[2u * k] += 1u;
+    else if (tg == 2 || tg == 4) c.hist[2u * k + 1u] += 1u;
+    c.beta_of[c.perm[k]] = c4(X.betas)[k % B];
+  }
   __syncthreads();
 }
 
-// 3. Metropolis sweep, quad levels
-__device__ __forceinline__ void sweep_phase(Ctx &c, uint32_t t) {
-  const PtGroup &X = *c.X;
+// local field of site i for the lane's replica (word g), summed in coupling_vecs order
+template <int D>
+__device__ __forceinline__ float site_field(const Ctx &c, uint32_t i, uint32_t g, uint32_t half, uint32_t sh) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t G = c.G;
+  float d = 0.0f;
+  d += c4(X.bias)[i];
+  if constexpr (D > 0) {
+    const TAMC_AS4 uint32_t *cp = c4(X.ell_col) + (size_t)i * D;
+    const TAMC_AS4 float *jp = c4(X.ell_J) + (size_t)i * D;
+    uint32_t w[D];
+    float Jv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      Jv[k] = jp[k];
+      w[k] = c.sp[(cp[k] * G + g) * 2u + half];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) d += ((w[k] >> sh) & 1u) ? -Jv[k] : Jv[k];
+  } else {
+    const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t j = c4(X.col)[e];
+      const float Je = c4(X.J)[e];
+      const uint32_t wj = c.sp[(j * G + g) * 2u + half];
+      d += ((wj >> sh) & 1u) ? -Je : Je;
+    }
+  }
+  return d;
+}
+
+// 3. Metropolis sweep, level by level
+template <int D>
+__device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
+  const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t G = c.G, n = c.n;
   long long acc[4] = {0, 0, 0, 0};
   const uint32_t half = c.lane >> 5, sh = c.lane & 31u;
   for (uint32_t l = 0; l < X.n_levels; ++l) {
-    const uint32_t a = X.lvl_off[l], b = X.lvl_off[l + 1];
+    const uint32_t a = c4(X.lvl_off)[l], b = c4(X.lvl_off)[l + 1];
     for (uint32_t k = a + c.wave; k < b; k += c.W) {
-      const uint32_t task = uni(X.tasks[k]);
-      const uint32_t quad = task / G, g = task - quad * G;
+      const uint32_t task = uni(c4(X.tasks)[k]);
+      const uint32_t grp = task / G, g = task - grp * G;
       const uint32_t r = g * 64u + c.lane;
       const bool valid = r < c.R;
       const float beta = valid ? c.beta_of[r] : 0.0f;
       const uint32_t rho = X.chain0 * c.B + r;
-      const U4 u = philox4x32_10(t, quad, rho, kTagSweep, c.key0, c.key1);
-      float a_g = 0.f;
-      const uint32_t i0 = quad * 4u;
+      const U4 u = philox4x32_10(t, grp, rho, kTagSweep, c.key0, c.key1);
+      uint32_t site[4];
+#pragma unroll
+      for (uint32_t s = 0; s < 4u; ++s) site[s] = c4(X.grp)[grp * 4u + s];
+      // the four sites of a group are mutually uncoupled: gather all fields first (independent
+      // LDS reads in flight together), then decide and write the new spin words
+      float d[4];
+      uint32_t bi[4];
 #pragma unroll
       for (uint32_t s = 0; s < 4u; ++s) {
-        const uint32_t i = i0 + s;
-        if (i >= n) break;
-        float d = 0.0f;
-        d += X.bias[i];
-        const uint32_t e0 = X.row_ptr[i], e1 = X.row_ptr[i + 1];
-        for (uint32_t e = e0; e < e1; ++e) {
-          const uint32_t j = X.col[e];
-          const float Je = X.J[e];
-          const uint32_t wj = c.sp[(j * G + g) * 2u + half];
-          d += ((wj >> sh) & 1u) ? -Je : Je;
+        d[s] = 0.0f;
+        bi[s] = 0u;
+        if (site[s] != kNone) {
+          d[s] = site_field<D>(c, site[s], g, half, sh);
+          bi[s] = (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u;
         }
-        const uint32_t wi = c.sp[(i * G + g) * 2u + half];
-        const uint32_t bi = (wi >> sh) & 1u;
-        d *= bi ? 2.0f : -2.0f;  // d *= -2*s_i
-        bool acc_ok = d < 0.0f;
-        if (!acc_ok) acc_ok = accept_u(-beta * d, pick(u, s));
-        acc_ok = acc_ok && valid;
-        const uint64_t nw = ballot((bi ^ (acc_ok ? 1u : 0u)) != 0u);
-        if (sh == 0u) c.sp[(i * G + g) * 2u + half] = half ? (uint32_t)(nw >> 32) : (uint32_t)nw;
-        if (acc_ok) a_g += d;
       }
-      // the quad's accepted dE, summed in site order, enter the fixed-point total
+      float a_g = 0.f;
+#pragma unroll
+      for (uint32_t s = 0; s < 4u; ++s) {
+        if (site[s] == kNone) break;
+        const uint32_t i = site[s];
+        float dd = d[s] * (bi[s] ? 2.0f : -2.0f);  // dE *= -2*s_i
+        bool acc_ok = dd < 0.0f;
+        if (!acc_ok) acc_ok = accept_u(-beta * dd, pick(u, s));
+        acc_ok = acc_ok && valid;
+        const uint64_t nw = ballot((bi[s] ^ (acc_ok ? 1u : 0u)) != 0u);
+        if (sh == 0u) c.sp[(i * G + g) * 2u + half] = half ? (uint32_t)(nw >> 32) : (uint32_t)nw;
+        if (acc_ok) a_g += dd;
+        if (X.self_J != nullptr) {
+          const float sj = c4(X.self_J)[i];
+          if (acc_ok && sj != 0.0f) atomicAdd((unsigned long long *)&c.drift[r], (unsigned long long)to_fix(-2.0f * sj));
+        }
+      }
+      // the group's accepted dE, summed in group order, enter the fixed-point total
       const long long fx = to_fix(a_g);
 #pragma unroll
       for (uint32_t gg = 0; gg < 4u; ++gg)
@@ -418,13 +560,20 @@ __device__ __forceinline__ void sweep_phase(Ctx &c, uint32_t t) {
   __syncthreads();
 }
 
+__device__ __forceinline__ void sweep_phase(Ctx &c, uint32_t t) {
+  const uint32_t D = c.X->ell_D;
+  if (D == 4u) sweep_phase_d<4>(c, t);
+  else if (D == 8u) sweep_phase_d<8>(c, t);
+  else sweep_phase_d<0>(c, t);
+}
+
 __device__ __forceinline__ uint32_t event_index(uint32_t stp, uint32_t S) {
   return (stp % S == 0u) ? stp / S : stp / S + 1u;
 }
 
 // 4. measurements (src/pt.rs:167-226) and ground-state tracking (src/pt.rs:559-581)
 __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
-  const PtGroup &X = *c.X;
+  const TAMC_AS4 PtGroup &X = *c.X;
   if (t < X.meas_init) return;
   const uint32_t stp = t - X.meas_init;
   const bool last = (t == X.T - 1u);
@@ -458,7 +607,7 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
           const uint32_t b = idx / X.nchi, k = idx % X.nchi;
           const uint32_t ra = c.perm[b], rb = c.perm[B + b];
           double x = 0.0;
-          const double *w = X.chi + (size_t)k * n;
+          const TAMC_AS4 double *w = c4(X.chi) + (size_t)k * n;
           for (uint32_t i = 0; i < n; ++i) {
             const double qi = (sbit(c.sp, G, i, ra) != sbit(c.sp, G, i, rb)) ? -1.0 : 1.0;
             x = __dadd_rn(x, __dmul_rn(w[i], qi));
@@ -531,9 +680,9 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
 // --------------------------------------------------------------------------------------------
 // Main kernel: work group blockIdx.x runs group groups[blockIdx.x] for sweeps [t0, t1).
 __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
-                                                        uint32_t icm_lds) {
+                                                        uint32_t icm_lds, uint32_t lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const PtGroup &X = groups[blockIdx.x];
+  const TAMC_AS4 PtGroup &X = *c4(groups + blockIdx.x);
   Ctx c;
   c.X = &X;
   c.n = X.n;
@@ -553,12 +702,16 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.key0 = (uint32_t)X.seed;
   c.key1 = (uint32_t)(X.seed >> 32);
   const bool use_lds_icm = icm_lds != 0u && !(X.flags & kFlagIcmGlobal);
-  const LdsLayout L = make_layout(c.n, c.G, c.R, c.B, c.P, c.W, use_lds_icm);
+  const LdsLayout L = with_nbr(make_layout(c.n, c.G, c.R, c.B, c.P, c.W, use_lds_icm), c.n,
+                               X.ell_col16 ? X.ell_D : 0u, lds_bytes);
   c.sp = (uint32_t *)(smem + L.spins);
   c.energy = (float *)(smem + L.energy);
   c.beta_of = (float *)(smem + L.beta_of);
   c.perm = (uint32_t *)(smem + L.perm);
   c.tags = (uint8_t *)(smem + L.tags);
+  c.nacc = (uint32_t *)(smem + L.nacc);
+  c.hist = (uint32_t *)(smem + L.hist);
+  c.rt = (uint32_t *)(smem + L.rt);
   c.pair_a = (uint32_t *)(smem + L.pair_a);
   c.pair_b = (uint32_t *)(smem + L.pair_b);
   c.pair_of = (uint32_t *)(smem + L.pair_of);
@@ -567,6 +720,9 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.misc = (uint32_t *)(smem + L.misc);
   c.partial = (long long *)(smem + L.partial);
   c.dEx = (float *)(smem + L.dE);
+  c.drift = (long long *)(smem + L.drift);
+  c.uw = (uint32_t *)(smem + L.uw);
+  c.dEa = (long long *)(smem + L.dEa);
   if (use_lds_icm && c.P > 0) {
     c.qw = (uint32_t *)(smem + L.qw);
     c.cw = (uint32_t *)(smem + L.cw);
@@ -576,32 +732,69 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     c.cw = X.g_cw;
     c.blk = X.g_blk;
   }
+  c.nbr = nullptr;
+  if (c.P > 0 && X.ell_col16 != nullptr && X.ell_D != 0u) {
+    if (L.nbr != kNone) {
+      uint16_t *nb = (uint16_t *)(smem + L.nbr);
+      for (uint32_t k = c.tid; k < c.n * X.ell_D; k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
+      c.nbr = nb;
+    } else {
+      c.nbr = X.ell_col16;
+    }
+  }
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
-  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) c.energy[r] = X.energy[r];
+  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
+    c.energy[r] = X.energy[r];
+    c.drift[r] = X.drift[r];
+  }
   for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) {
     c.perm[k] = X.perm[k];
     c.tags[k] = X.tags[k];
+    c.nacc[k] = X.nacc[k];
+    c.hist[2 * k] = X.hist[2 * k];
+    c.hist[2 * k + 1] = X.hist[2 * k + 1];
   }
+  if (c.tid < c.nch) c.rt[c.tid] = X.rt[c.tid];
   __syncthreads();
-  for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) c.beta_of[c.perm[k]] = X.betas[k % c.B];
+  for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) c.beta_of[c.perm[k]] = c4(X.betas)[k % c.B];
   __syncthreads();
   const uint32_t tend = t1 < X.T ? t1 : X.T;
+  unsigned long long ticks[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long tk = wall_clock64();
+  auto lap = [&](int ph) {
+    const unsigned long long now = wall_clock64();
+    ticks[ph] += now - tk;
+    tk = now;
+  };
   for (uint32_t t = t0; t < tend; ++t) {
-    if ((c.flags & kFlagIcm) && c.P > 0) icm_phase(c, t);
+    if ((c.flags & kFlagIcm) && c.P > 0) icm_phase(c, t, ticks, tk);
+    lap(2);
     if (c.flags & kFlagIcmOnly) break;
     exchange_phase(c, t);
+    lap(3);
     sweep_phase(c, t);
+    lap(4);
     measure_phase(c, t);
+    lap(5);
   }
+  if (c.tid == 0 && X.phase_ticks)
+    for (int k = 0; k < 6; ++k) X.phase_ticks[k] += ticks[k];
   // store persistent state
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
-  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) X.energy[r] = c.energy[r];
+  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
+    X.energy[r] = c.energy[r];
+    X.drift[r] = c.drift[r];
+  }
   for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) {
     X.perm[k] = c.perm[k];
     X.tags[k] = c.tags[k];
+    X.nacc[k] = c.nacc[k];
+    X.hist[2 * k] = c.hist[2 * k];
+    X.hist[2 * k + 1] = c.hist[2 * k + 1];
   }
+  if (c.tid < c.nch) X.rt[c.tid] = c.rt[c.tid];
 }
 
 }  // namespace tamc
@@ -615,11 +808,15 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
 
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
+  // dynamic LDS above 64 KiB must be requested explicitly, at the size the launch uses
+  static uint32_t attr_bytes = 65536u;
+  if (lds_bytes > attr_bytes) {
+    const hipError_t e = hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    attr_bytes = lds_bytes;
   }
-  hipLaunchKernelGGL(tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1, icm_lds);
+  hipLaunchKernelGGL(tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1, icm_lds,
+                     lds_bytes);
   return hipGetLastError();
 }

@@ -119,7 +119,9 @@ static tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tam
 }
 
 static uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
-  return make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global).total;
+  return with_nbr(make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global), S.n,
+                  g.g.ell_col16 ? g.g.ell_D : 0u, kLdsLimit)
+      .total;
 }
 
 // Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
@@ -131,6 +133,8 @@ static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
       const uint32_t nb = (S.n + 63) / 64;
       A.want(&X.spins, (size_t)S.n * g.G * 2);
       A.want(&X.energy, g.R);
+      A.want(&X.drift, g.R);
+      A.want(&X.phase_ticks, 6);
       A.want(&X.perm, (size_t)g.nch * S.B);
       A.want(&X.tags, (size_t)g.nch * S.B);
       A.want(&X.nacc, (size_t)g.nch * S.B);
@@ -170,6 +174,11 @@ static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
       X.g_row_ptr = (const uint32_t *)D->g_row_ptr.p;
       X.g_col = (const uint32_t *)D->g_col.p;
       X.chi = (const double *)D->chi.p;
+      X.ell_col = (const uint32_t *)D->ell_col.p;
+      X.ell_col16 = S.n < 65536u ? (const uint16_t *)D->ell_col16.p : nullptr;
+      X.ell_J = (const float *)D->ell_J.p;
+      X.ell_D = D->ell_D;
+      X.self_J = D->has_self ? (const float *)D->self_J.p : nullptr;
       X.offset = S.inst->offset;
       X.n = S.n;
       X.g_nodes = S.inst->g_nodes;
@@ -178,6 +187,7 @@ static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
       X.lvl_off = (const uint32_t *)Sc->lvl_off.p;
       X.tasks = (const uint32_t *)Sc->tasks.p;
       X.n_levels = Sc->n_levels;
+      X.grp = (const uint32_t *)Sc->groups.p;
       X.betas = S.d_betas;
       X.B = S.B;
       X.nch = g.nch;
@@ -481,6 +491,19 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
     attempts += results[k]->attempts;
   }
   if (stats) {
+    // phase split of the group that took longest
+    unsigned long long best[6] = {0, 0, 0, 0, 0, 0}, best_tot = 0;
+    for (auto &S : solves)
+      for (auto &g : S.groups) {
+        unsigned long long tk[6];
+        if (hipMemcpy(tk, g.g.phase_ticks, sizeof tk, hipMemcpyDeviceToHost) != hipSuccess) continue;
+        const unsigned long long tot = tk[0] + tk[1] + tk[2] + tk[3] + tk[4] + tk[5];
+        if (tot > best_tot) {
+          best_tot = tot;
+          memcpy(best, tk, sizeof tk);
+        }
+      }
+    for (int k = 0; k < 6; ++k) stats->phase_ms[k] = (double)best[k] * 1e-5;  // 100 MHz ticks
     stats->device_ms = ms;
     stats->wall_ms = wall_us / 1000.0;
     stats->attempts = attempts;
@@ -494,7 +517,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
 
 static uint32_t default_waves(const tamc_instance *inst) {
   // enough waves to cover the typical level width, within one work group
-  const uint32_t nq = (uint32_t)inst->quad_level.size();
+  const uint32_t nq = inst->n_groups;
   const uint32_t width = inst->n_levels ? (nq + inst->n_levels - 1) / inst->n_levels : 1;
   uint32_t w = 4;
   while (w < width && w < 16) w *= 2;

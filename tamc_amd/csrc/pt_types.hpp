@@ -35,9 +35,10 @@ struct PtGroup {
   const double *chi;         // [nchi][n]
   float offset;
   uint32_t n, g_nodes, nchi, nblk;
-  // ---- sequential-order schedule: levels of site quads ----
+  // ---- sequential-order schedule: levels of mutually uncoupled sites, in groups of four ----
   const uint32_t *lvl_off;   // [n_levels+1] offsets into tasks
-  const uint32_t *tasks;     // quad*G + g
+  const uint32_t *tasks;     // group*G + g
+  const uint32_t *grp;       // [n_groups][4] sites of each group (kNone pads)
   uint32_t n_levels;
   // ---- ladder ----
   const float *betas;        // [B]
@@ -70,11 +71,24 @@ struct PtGroup {
   uint32_t *g_qw, *g_cw, *g_blk;
   // ---- kFlagIcmOnly: seed index per pair (into the ascending q<0 list) ----
   const uint32_t *icm_k;
+  // ---- fixed-stride copy of coupling_vecs for rows of at most ell_D entries (0 = use CSR) ----
+  const uint32_t *ell_col;   // [n][ell_D], padding entries point at the row's own site
+  const uint16_t *ell_col16; // the same as 16-bit indices (n < 65536), cached in LDS for ICM
+  const float *ell_J;        // [n][ell_D], padding entries are +0.0
+  uint32_t ell_D, pad2;
+  // ---- QUBO self entries (src/ising.rs:308-310): J_ii per site, null when the instance has none.
+  // Their flips shift the reference's tracked energy by -2 J_ii each until energy_ref is recomputed
+  // after an ICM move; `drift` carries that shift (2^-32 fixed point) per replica.
+  const float *self_J;
+  long long *drift;          // [R]
+  // ---- per-phase time of the group (100 MHz wall clock ticks): ICM overlap+seeds, ICM cluster growth,
+  //      ICM flip, exchange, sweep, measurement ----
+  unsigned long long *phase_ticks;  // [6]
 };
 
 struct LdsLayout {
-  uint32_t spins, energy, beta_of, perm, tags, pair_a, pair_b, pair_of, seed_blk, seed_rank, misc, partial, dE, qw,
-      cw, blk, total;
+  uint32_t spins, energy, beta_of, perm, tags, nacc, hist, rt, pair_a, pair_b, pair_of, seed_blk, seed_rank, misc, partial, dE, drift,
+      uw, dEa, qw, cw, blk, nbr, total;
 };
 
 __host__ __device__ inline uint32_t lds_align(uint32_t x) { return (x + 15u) & ~15u; }
@@ -95,6 +109,12 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(2u * B * 4u);
   L.tags = o;
   o += lds_align(2u * B);
+  L.nacc = o;
+  o += lds_align(2u * B * 4u);
+  L.hist = o;
+  o += lds_align(4u * B * 4u);
+  L.rt = o;
+  o += lds_align(2u * 4u);
   L.pair_a = o;
   o += lds_align(kMaxPairs * 4u);
   L.pair_b = o;
@@ -111,6 +131,12 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(W * G * 64u * 8u);
   L.dE = o;  // exchange energy differences [2][B]
   o += lds_align(2u * B * 4u);
+  L.drift = o;
+  o += lds_align(R * 8u);
+  L.uw = o;  // exchange uniforms [2][B]
+  o += lds_align(2u * B * 4u);
+  L.dEa = o;  // ICM energy change of the first replica of each pair (fixed point)
+  o += lds_align(kMaxPairs * 8u);
   if (icm_lds && P > 0) {
     L.qw = o;
     o += lds_align(n * 4u);
@@ -121,7 +147,18 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   } else {
     L.qw = L.cw = L.blk = kNone;
   }
+  L.nbr = kNone;
   L.total = o;
+  return L;
+}
+
+// Adds the LDS copy of the 16-bit neighbour lists used by ICM when it fits under `limit`.
+__host__ __device__ inline LdsLayout with_nbr(LdsLayout L, uint32_t n, uint32_t D, uint32_t limit) {
+  if (D == 0u || L.qw == kNone) return L;
+  const uint32_t need = lds_align(n * D * 2u);
+  if (L.total + need > limit) return L;
+  L.nbr = L.total;
+  L.total += need;
   return L;
 }
 

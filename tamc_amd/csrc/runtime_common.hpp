@@ -73,12 +73,14 @@ struct Arena {
 };
 
 struct DeviceSchedule {
-  DevBuf lvl_off, tasks;
+  DevBuf lvl_off, tasks, groups;
   uint32_t n_levels = 0, n_tasks = 0;
 };
 
 struct DeviceInstance {
-  DevBuf row_ptr, col, J, bias, g_row_ptr, g_col, chi;
+  DevBuf row_ptr, col, J, bias, g_row_ptr, g_col, chi, ell_col, ell_col16, ell_J, self_J;
+  bool has_self = false;
+  uint32_t ell_D = 0;
   std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by words per site G
 };
 
@@ -93,8 +95,13 @@ struct tamc_instance {
   std::vector<uint32_t> g_row_ptr, g_col;  // coupling CsMat pattern (ICM graph)
   uint32_t g_nodes = 0;
   std::vector<std::vector<double>> chi;
-  std::vector<uint32_t> quad_level;  // sequential-order level of every site quad
-  uint32_t n_levels = 0;
+  // Sequential-order schedule: level(i) = 1 + max level of the lower-numbered sites coupled to i
+  // (0 if none); sites of one level are mutually uncoupled.  Within a level, sites in ascending
+  // order form groups of four ("RNG groups", numbered level-major); one Philox call keyed by the
+  // group serves its four sites.
+  std::vector<uint32_t> group_sites;  // [n_groups][4], kNone pads the last group of a level
+  std::vector<uint32_t> level_groups; // [n_levels+1] offsets into groups
+  uint32_t n_levels = 0, n_groups = 0;
   std::mutex mu;
   std::unique_ptr<tamc::DeviceInstance> dev;
 };

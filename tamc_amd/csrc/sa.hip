@@ -169,10 +169,11 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     }
   }
   TAMC_HIP(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SaTask), hipMemcpyHostToDevice));
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void *)sa_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
+  static uint32_t attr_bytes = 65536u;
+  const uint32_t lds = wpg * wpw * 4u;
+  if (lds > attr_bytes) {
+    TAMC_HIP(hipFuncSetAttribute((const void *)sa_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_bytes = lds;
   }
   hipEvent_t e0, e1;
   TAMC_HIP(hipEventCreate(&e0));

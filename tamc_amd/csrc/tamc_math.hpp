@@ -52,8 +52,11 @@ TAMC_HD U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uin
   return U4{c0, c1, c2, c3};
 }
 
+// word (lane & 3) of v, computed with masks so that a run-time lane never becomes a memory lookup
 TAMC_HD uint32_t pick(const U4 &v, uint32_t lane) {
-  return (lane & 2) ? ((lane & 1) ? v.w : v.z) : ((lane & 1) ? v.y : v.x);
+  const uint32_t l = lane & 3u;
+  return (v.x & (0u - (uint32_t)(l == 0u))) | (v.y & (0u - (uint32_t)(l == 1u))) |
+         (v.z & (0u - (uint32_t)(l == 2u))) | (v.w & (0u - (uint32_t)(l == 3u)));
 }
 
 // rand 0.8 Standard f32 from one 32-bit word, as an integer numerator over 2^24.
