@@ -170,17 +170,56 @@ struct SiteWords {
   }
 };
 
+// Wave-wide sums and ORs with DPP row shifts and row broadcasts (an inclusive scan whose last lane
+// holds the total); every lane of the wave must be active.
+#define TAMC_DPP_STEP64(ctrl, rmask)                                                   \
+  {                                                                                    \
+    const uint32_t l2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rmask, 0xf, false); \
+    const uint32_t h2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rmask, 0xf, false); \
+    const uint64_t s = (((uint64_t)hi << 32) | lo) + (((uint64_t)h2 << 32) | l2);      \
+    lo = (uint32_t)s;                                                                  \
+    hi = (uint32_t)(s >> 32);                                                          \
+  }
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  TAMC_DPP_STEP64(0x111, 0xf)  // row_shr:1
+  TAMC_DPP_STEP64(0x112, 0xf)  // row_shr:2
+  TAMC_DPP_STEP64(0x114, 0xf)  // row_shr:4
+  TAMC_DPP_STEP64(0x118, 0xf)  // row_shr:8
+  TAMC_DPP_STEP64(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+  TAMC_DPP_STEP64(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
+  const uint32_t tl = __builtin_amdgcn_readlane(lo, 63), th = __builtin_amdgcn_readlane(hi, 63);
+  return ((uint64_t)th << 32) | tl;
+}
+#undef TAMC_DPP_STEP64
+#define TAMC_DPP_OR(ctrl, rmask) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, rmask, 0xf, false);
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+  TAMC_DPP_OR(0x111, 0xf)
+  TAMC_DPP_OR(0x112, 0xf)
+  TAMC_DPP_OR(0x114, 0xf)
+  TAMC_DPP_OR(0x118, 0xf)
+  TAMC_DPP_OR(0x142, 0xa)
+  TAMC_DPP_OR(0x143, 0xc)
+  return __builtin_amdgcn_readlane(v, 63);
+}
+#undef TAMC_DPP_OR
+
 // ICM step (d) over sites, lane = site.  For fixed-stride rows (D > 0) the site's couplings and its
-// neighbours' cluster words are loaded once and reused for every pair the site belongs to.
+// neighbours' cluster words are loaded once and reused for every pair the site belongs to.  Pair p only reads neighbours outside its cluster and only flips
+// replicas a_p, b_p of sites inside it, so waves may flip their blocks as soon as they are done.
 template <int D>
 __device__ __forceinline__ void icm_flip(Ctx &c) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t P = c.P, n = c.n, G = c.G;
   constexpr int DD = D > 0 ? D : 1;
+  // replica indices of every pair, lane p holding pair p
+  const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
+  const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
     const uint32_t i = blk * 64u + c.lane;
     const uint32_t cwi = (i < n) ? c.cw[i] : 0u;
-    if (ballot(cwi != 0u) == 0ull) continue;
+    uint32_t act = wave_or_u32(cwi);
+    if (act == 0u) continue;
     SiteWords h{0ull, c.sp, 0u, G};
     unsigned long long M[4] = {0, 0, 0, 0};
     float bias_i = 0.f;
@@ -203,18 +242,21 @@ __device__ __forceinline__ void icm_flip(Ctx &c) {
         }
 #pragma unroll
         for (int k = 0; k < D; ++k) ncw[k] = c.cw[nj[k]];
+
       }
     }
-    for (uint32_t p = 0; p < P; ++p) {
+    while (act) {
+      const uint32_t p = (uint32_t)__builtin_ctz(act);
+      act &= act - 1u;
+      const uint32_t a = __builtin_amdgcn_readlane(pa_l, p), b = __builtin_amdgcn_readlane(pb_l, p);
       const uint32_t bp = (cwi >> p) & 1u;
-      if (ballot(bp != 0u) == 0ull) continue;
-      const uint32_t a = c.pair_a[p], b = c.pair_b[p];
+      long long v = 0;
       if (bp) {
         float hh = bias_i;
         if constexpr (D > 0) {
           uint32_t sj[D];
 #pragma unroll
-          for (int k = 0; k < D; ++k) sj[k] = sbit(c.sp, G, nj[k], a);
+            for (int k = 0; k < D; ++k) sj[k] = sbit(c.sp, G, nj[k], a);
 #pragma unroll
           for (int k = 0; k < D; ++k)
             if (!((ncw[k] >> p) & 1u)) hh += sj[k] ? -nJ[k] : nJ[k];
@@ -229,13 +271,15 @@ __device__ __forceinline__ void icm_flip(Ctx &c) {
           }
         }
         const float term = h.bit(a) ? 2.0f * hh : -2.0f * hh;
-        atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)to_fix(term));
+        v = to_fix(term);
 #pragma unroll
         for (uint32_t g = 0; g < 4u; ++g) {
           if ((a >> 6) == g) M[g] |= 1ull << (a & 63u);
           if ((b >> 6) == g) M[g] |= 1ull << (b & 63u);
         }
       }
+      const uint64_t tot = wave_sum_u64((uint64_t)v);
+      if (c.lane == 0u) atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)tot);
     }
 #pragma unroll
     for (uint32_t g = 0; g < 4u; ++g)
