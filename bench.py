@@ -27,6 +27,17 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_ATTEMPT_D3 = 9 * 3 + 9  # 4 B col + 4 B J + 1 B spin per edge; 4 B row_ptr + 4 B bias + 1 B spin per site
 
 
+def measured_traffic():
+    """HBM bytes per launch of pt_sweep_kernel on this workload, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs recorded beside the kernel-trace summary (scaled by the copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "pt_sweep_kernel_rr3x256_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def build_workload(count, seed, T):
     from tamc_amd import api, workloads
     insts, lines = [], []
@@ -67,13 +78,28 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--sweeps", type=int, default=4000)
     ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--cpu-sweeps", type=int, default=100)
-    ap.add_argument("--cpu-instances", type=int, default=64)
+    ap.add_argument("--cpu-sweeps", type=int, default=200)
+    ap.add_argument("--cpu-instances", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only-largest", action="store_true", help="run only the largest instance (profiling)")
+    ap.add_argument("--no-icm", action="store_true", help="switch ICM off (profiling the sweep alone)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (float32 elementwise), the reference point "
+                         "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration, "read_bytes": 4 * n, "write_bytes": 4 * n}))
+        return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (independent instances would shard across GPUs with "
               "no communication); --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -83,6 +109,12 @@ def main():
     from tamc_amd._lib import check, lib
     check(lib().tamc_init())
     insts, lines, params = build_workload(args.instances, 103, args.sweeps)
+    if args.only_largest:
+        k = int(np.argmax([x.n for x in insts]))
+        insts, lines, params = [insts[k]], [lines[k]], [params[k]]
+    if args.no_icm:
+        for p in params:
+            p.icm = False
     sum_n = sum(x.n for x in insts)
     attempts_per_step = sum(p.num_sweeps * x.n * 32 * 2 for p, x in zip(params, insts))
 
@@ -104,14 +136,18 @@ def main():
     ms_per_step = wall / args.steps * 1e3
 
     # plain path: one instance per launch, sequentially, on a sample of instances
+    plain_rate = None
     k = min(args.plain_sample, len(insts))
-    _, pst = api.run_pt_batch(insts[:1], params[:1], plain=True)  # warm
-    t1 = time.perf_counter()
-    _, pst = api.run_pt_batch(insts[:k], params[:k], plain=True)
-    plain_wall = time.perf_counter() - t1
-    plain_rate = pst["attempts"] / plain_wall
+    if not args.no_plain and k > 0:
+        _, pst = api.run_pt_batch(insts[:1], params[:1], plain=True)  # warm
+        t1 = time.perf_counter()
+        _, pst = api.run_pt_batch(insts[:k], params[:k], plain=True)
+        plain_wall = time.perf_counter() - t1
+        plain_rate = pst["attempts"] / plain_wall
 
     achieved = kernel_rate * BYTES_PER_ATTEMPT_D3 / 1e9
+    default_workload = (args.instances == 256 and args.sweeps == 4000 and not args.only_largest
+                        and not args.no_icm)
     out = {
         "metric": "spin-flip attempts/s per GPU (batched PT-ICM)",
         "value": value,
@@ -136,10 +172,12 @@ def main():
         "kernel_attempts_per_s": kernel_rate,
         "plain_gpu": {"value": plain_rate, "unit": "spin-flip attempts/s", "instances": k,
                       "note": "one instance per kernel launch, launched one after another"},
-        "batched_vs_plain": value / plain_rate,
+        "batched_vs_plain": (value / plain_rate) if plain_rate else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_attempt": BYTES_PER_ATTEMPT_D3},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic() if default_workload else None,
+                     "bytes_per_attempt": BYTES_PER_ATTEMPT_D3,
+                     "algorithmic_bytes_per_launch": attempts_per_step * BYTES_PER_ATTEMPT_D3,
+                     "kernel_ms": float(np.mean(dev_ms))},
     }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -149,6 +187,9 @@ def main():
                                "sample": "%d instances x %d sweeps of the same workload (%.3g attempts, %.1f s)"
                                          % (min(args.cpu_instances, len(lines)), args.cpu_sweeps, att, dt)}
         out["batched_vs_cpu"] = value / rate
+        rate1, att1, dt1 = cpu_baseline(lines[:16], params[:16], 50, 1)
+        out["cpu_baseline_1thread"] = {"value": rate1, "unit": "spin-flip attempts/s", "cores": 1,
+                                       "sample": "16 instances x 50 sweeps (%.3g attempts, %.1f s)" % (att1, dt1)}
     print(json.dumps(out))
     return 0
 

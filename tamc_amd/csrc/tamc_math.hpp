@@ -1,7 +1,7 @@
 // tamc_math.hpp -- arithmetic shared by the gfx950 kernels and the host runtime.
 //
 //  * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based generator that gives
-//    every (sweep, site quad, replica) its own four 32-bit words.
+//    every (sweep, group of four mutually uncoupled sites, replica) its own four 32-bit words.
 //  * tamc_expf_cr: exp of a float evaluated in double precision (Cody-Waite reduction by ln 2,
 //    degree-13 Taylor polynomial in Horner form with explicit fma, exact scaling by 2^k) and rounded
 //    once to float.  The reference's acceptance test uses f32::exp, i.e. glibc expf; the runtime
