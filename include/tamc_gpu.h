@@ -28,7 +28,8 @@ typedef enum {
   TAMC_E_DEVICE = 3,      /* HIP runtime error or no gfx950 device */
   TAMC_E_NOMEM = 4,
   TAMC_E_UNSUPPORTED = 5, /* shape outside what the kernels accept (see tamc_last_error) */
-  TAMC_E_STREAM = 6       /* lock-step uniform stream exhausted */
+  TAMC_E_STREAM = 6,      /* lock-step uniform stream exhausted */
+  TAMC_E_PENDING = 7      /* tamc_server_poll: the request has not finished yet */
 } tamc_status;
 
 /* Last error message of the calling thread ("" if none). */
@@ -177,6 +178,54 @@ tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts,
 /* The plain path: the same solves, one instance per kernel launch, one after another. */
 tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               tamc_pt_result **results, tamc_batch_stats *stats);
+
+/* ------------------------------------------------------------- serving ---- */
+/* A request loop with continuous batching.  The reference has no server: its callers run many
+ * PtIcmRunner solves concurrently from rayon (src/gla.rs:152-162, src/bin/tamc-pt-opt.rs:68-71);
+ * a server takes the place of those concurrent runner calls.  Requests are admitted while device
+ * memory (a pool reserved at create) and work-group slots allow; admitted requests stay resident
+ * on the device and advance in time slices that all resident requests share, so requests that
+ * arrive while others run join at the next slice.  Each result is identical to tamc_pt_run on
+ * the same (instance, params, seed).  All functions are thread-safe; the instance passed to
+ * tamc_server_submit must stay alive until the request has finished or been cancelled. */
+typedef struct tamc_server tamc_server;
+
+typedef struct {
+  uint64_t pool_bytes;       /* device memory reserved for request state (0 = 4 GiB) */
+  uint32_t max_wave_slots;   /* waves of resident work groups, summed (0 = 16 per compute unit) */
+  double slice_ms;           /* target device time of one slice (0 = 4 ms) */
+  uint32_t waves_per_group;  /* 0 = per request from its level width (4, 8 or 16) */
+} tamc_server_options;
+
+typedef struct {
+  double queued_ms;          /* submit -> admitted */
+  double total_ms;           /* submit -> result ready on the host */
+  uint32_t slices;           /* slices the request took part in */
+} tamc_request_times;
+
+typedef struct {
+  uint64_t submitted, completed, failed, cancelled;
+  uint32_t queued, resident, resident_waves;
+  uint64_t slices;
+  double device_busy_ms;     /* sum of slice device times */
+  uint64_t attempts;         /* Metropolis attempts of completed requests */
+  uint64_t pool_used, pool_bytes;
+} tamc_server_stats;
+
+tamc_status tamc_server_create(const tamc_server_options *opts, tamc_server **out);
+/* Validates the request like tamc_pt_run and queues it; *ticket identifies it. */
+tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, const tamc_pt_params *params,
+                               const int8_t *init_states, uint64_t *ticket);
+/* wait_ms: 0 = do not wait, < 0 = wait until done.  TAMC_OK and *out = the result once the request
+ * has finished (the ticket is then released; free the result with tamc_pt_result_free);
+ * TAMC_E_PENDING while it runs; the request's own error if it failed. */
+tamc_status tamc_server_poll(tamc_server *srv, uint64_t ticket, int32_t wait_ms, tamc_pt_result **out,
+                             tamc_request_times *times);
+/* Drops a queued or resident request (its device memory is released at the next slice boundary). */
+tamc_status tamc_server_cancel(tamc_server *srv, uint64_t ticket);
+tamc_status tamc_server_get_stats(tamc_server *srv, tamc_server_stats *stats);
+/* Cancels what is left, stops the loop and releases the device pool. */
+void tamc_server_destroy(tamc_server *srv);
 
 /* ---------------------------------------------------------------------- SA ---- */
 /* SaParams (src/sa.rs:21-26) with BetaOptions resolved to the schedule array. */

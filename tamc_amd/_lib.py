@@ -10,8 +10,9 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libtamc_gpu.so")
 
-TAMC_OK, TAMC_E_INVALID_ARG, TAMC_E_IO, TAMC_E_DEVICE, TAMC_E_NOMEM, TAMC_E_UNSUPPORTED, TAMC_E_STREAM = range(7)
-_STATUS = {1: "INVALID_ARG", 2: "IO", 3: "DEVICE", 4: "NOMEM", 5: "UNSUPPORTED", 6: "STREAM"}
+(TAMC_OK, TAMC_E_INVALID_ARG, TAMC_E_IO, TAMC_E_DEVICE, TAMC_E_NOMEM, TAMC_E_UNSUPPORTED, TAMC_E_STREAM,
+ TAMC_E_PENDING) = range(8)
+_STATUS = {1: "INVALID_ARG", 2: "IO", 3: "DEVICE", 4: "NOMEM", 5: "UNSUPPORTED", 6: "STREAM", 7: "PENDING"}
 
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
@@ -78,6 +79,22 @@ class SaResult(C.Structure):
                 ("device_ms", C.c_double)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("pool_bytes", C.c_uint64), ("max_wave_slots", C.c_uint32), ("slice_ms", C.c_double),
+                ("waves_per_group", C.c_uint32)]
+
+
+class RequestTimes(C.Structure):
+    _fields_ = [("queued_ms", C.c_double), ("total_ms", C.c_double), ("slices", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("cancelled", C.c_uint64), ("queued", C.c_uint32), ("resident", C.c_uint32),
+                ("resident_waves", C.c_uint32), ("slices", C.c_uint64), ("device_busy_ms", C.c_double),
+                ("attempts", C.c_uint64), ("pool_used", C.c_uint64), ("pool_bytes", C.c_uint64)]
+
+
 _vp = C.c_void_p
 _vpp = C.POINTER(C.c_void_p)
 SIGNATURES = {
@@ -106,6 +123,13 @@ SIGNATURES = {
                                     C.POINTER(C.POINTER(PtResult)), C.POINTER(BatchStats)]),
     "tamc_pt_run_plain": (C.c_int, [C.c_uint32, _vpp, C.POINTER(PtParams), C.POINTER(C.POINTER(PtResult)),
                                     C.POINTER(BatchStats)]),
+    "tamc_server_create": (C.c_int, [C.POINTER(ServerOptions), _vpp]),
+    "tamc_server_submit": (C.c_int, [_vp, _vp, C.POINTER(PtParams), _i8p, _u64p]),
+    "tamc_server_poll": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.POINTER(C.POINTER(PtResult)),
+                                   C.POINTER(RequestTimes)]),
+    "tamc_server_cancel": (C.c_int, [_vp, C.c_uint64]),
+    "tamc_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
+    "tamc_server_destroy": (None, [_vp]),
     "tamc_sa_run": (C.c_int, [_vp, C.POINTER(SaParams), C.POINTER(C.POINTER(SaResult))]),
     "tamc_sa_run_batch": (C.c_int, [C.c_uint32, _vpp, C.POINTER(SaParams), C.POINTER(C.POINTER(SaResult)),
                                     C.POINTER(BatchStats)]),

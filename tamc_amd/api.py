@@ -12,8 +12,9 @@ from typing import List, Optional
 
 import numpy as np
 
-from ._lib import (BatchOptions, BatchStats, InstanceInfo, PtParams as _CPt, PtResult, SaParams as _CSa, SaResult,
-                   check, lib, ptr, _f32p, _f64p, _i8p, _i64p, _u32p, _u64p, _u8p)
+from ._lib import (BatchOptions, BatchStats, InstanceInfo, PtParams as _CPt, PtResult, RequestTimes, SaParams as _CSa,
+                   SaResult, ServerOptions, ServerStats, TAMC_E_PENDING, check, lib, ptr, _f32p, _f64p, _i8p, _i64p,
+                   _u32p, _u64p, _u8p)
 
 
 class BqmIsingInstance:
@@ -219,14 +220,17 @@ class PtIcmRunner:
     def _c(self):
         return self.params.to_c(self.betas, self.lo_beta_idx)
 
-    def run(self, init_states=None):
-        """Production path; returns the result dictionary."""
+    def run(self, init_states=None, sample_output=None):
+        """Production path; returns the result dictionary.  With sample_output, the
+        PtIcmThermalSamples are also written there as bincode (src/pt.rs:640-654)."""
         P = self._c()
         out = C.POINTER(PtResult)()
         init = None if init_states is None else np.ascontiguousarray(init_states, np.int8)
         check(lib().tamc_pt_run(self.instance._h, C.byref(P), None if init is None else ptr(init, _i8p),
                                 C.byref(out)))
         try:
+            if sample_output is not None:
+                check(lib().tamc_write_thermal_samples(out, ptr(self.betas, _f32p), 0, str(sample_output).encode()))
             return pt_result_to_dict(out.contents)
         finally:
             lib().tamc_pt_result_free(out)
@@ -247,9 +251,6 @@ class PtIcmRunner:
             return pt_result_to_dict(out.contents)
         finally:
             lib().tamc_pt_result_free(out)
-
-    def write_thermal_samples(self, result_ptr, path):
-        raise NotImplementedError
 
 
 def _batch_args(instances, params_list):
@@ -282,6 +283,73 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms))
     return res, st
+
+
+class Server:
+    """Continuous-batching PT-ICM request loop (tamc_server_* in the C header).
+
+    submit() queues a solve and returns a ticket; poll() returns (result, times) once it has
+    finished, or None while it runs.  Instances must outlive their requests."""
+
+    def __init__(self, pool_bytes=0, max_wave_slots=0, slice_ms=0.0, waves_per_group=0):
+        self._h = C.c_void_p()
+        opt = ServerOptions(pool_bytes, max_wave_slots, slice_ms, waves_per_group)
+        check(lib().tamc_server_create(C.byref(opt), C.byref(self._h)))
+        self._keep = {}
+
+    def submit(self, instance, params, init_states=None):
+        betas, lo = params.resolve()
+        P = params.to_c(betas, lo)
+        init = None if init_states is None else np.ascontiguousarray(init_states, np.int8)
+        t = C.c_uint64()
+        check(lib().tamc_server_submit(self._h, instance._h, C.byref(P), None if init is None else ptr(init, _i8p),
+                                       C.byref(t)))
+        self._keep[t.value] = instance
+        return t.value
+
+    def poll(self, ticket, wait_ms=0):
+        out = C.POINTER(PtResult)()
+        tm = RequestTimes()
+        st = lib().tamc_server_poll(self._h, ticket, int(wait_ms), C.byref(out), C.byref(tm))
+        if st == TAMC_E_PENDING:
+            return None
+        self._keep.pop(ticket, None)
+        check(st)
+        try:
+            res = pt_result_to_dict(out.contents)
+        finally:
+            lib().tamc_pt_result_free(out)
+        return res, dict(queued_ms=tm.queued_ms, total_ms=tm.total_ms, slices=tm.slices)
+
+    def wait(self, ticket):
+        return self.poll(ticket, wait_ms=-1)
+
+    def cancel(self, ticket):
+        check(lib().tamc_server_cancel(self._h, ticket))
+        self._keep.pop(ticket, None)
+
+    def stats(self):
+        s = ServerStats()
+        check(lib().tamc_server_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().tamc_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

@@ -17,6 +17,8 @@
 // tiling or of batch placement.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "pt_types.hpp"
 #include "tamc_math.hpp"
 
@@ -723,10 +725,19 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
 
 // --------------------------------------------------------------------------------------------
 // Main kernel: work group blockIdx.x runs group groups[blockIdx.x] for sweeps [t0, t1).
+// One work group per PtGroup.  Sweeps [t0, t1) of every group, or, when `ranges` is given,
+// sweeps [ranges[2b], ranges[2b+1]) of group b (time slices of a serving loop).  When
+// `launch_ticks` is given, thread 0 stores the group's 100 MHz tick count for this launch there.
 __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
-                                                        uint32_t icm_lds, uint32_t lds_bytes) {
+                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
+                                                        unsigned long long *launch_ticks) {
   extern __shared__ __align__(16) unsigned char smem[];
   const TAMC_AS4 PtGroup &X = *c4(groups + blockIdx.x);
+  if (ranges) {
+    t0 = c4(ranges)[2u * blockIdx.x];
+    t1 = c4(ranges)[2u * blockIdx.x + 1u];
+  }
+  const unsigned long long launch_start = wall_clock64();
   Ctx c;
   c.X = &X;
   c.n = X.n;
@@ -825,6 +836,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   }
   if (c.tid == 0 && X.phase_ticks)
     for (int k = 0; k < 6; ++k) X.phase_ticks[k] += ticks[k];
+  if (c.tid == 0 && launch_ticks) launch_ticks[blockIdx.x] = wall_clock64() - launch_start;
   // store persistent state
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
@@ -851,16 +863,22 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
 }
 
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
-                                            uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, hipStream_t s) {
-  // dynamic LDS above 64 KiB must be requested explicitly, at the size the launch uses
+                                            uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
+                                            unsigned long long *launch_ticks, hipStream_t s) {
+  // dynamic LDS above 64 KiB must be requested explicitly; the attribute is raised to the largest
+  // size any launch has used (callers may launch from several threads)
+  static std::mutex mu;
   static uint32_t attr_bytes = 65536u;
-  if (lds_bytes > attr_bytes) {
-    const hipError_t e = hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-    if (e != hipSuccess) return e;
-    attr_bytes = lds_bytes;
+  if (lds_bytes > 65536u) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (lds_bytes > attr_bytes) {
+      const hipError_t e = hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      if (e != hipSuccess) return e;
+      attr_bytes = lds_bytes;
+    }
   }
   hipLaunchKernelGGL(tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1, icm_lds,
-                     lds_bytes);
+                     lds_bytes, ranges, launch_ticks);
   return hipGetLastError();
 }

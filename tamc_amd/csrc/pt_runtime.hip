@@ -12,42 +12,9 @@
 #include <string>
 #include <vector>
 
-#include "pt_types.hpp"
-#include "runtime_common.hpp"
-
-extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t ngroups,
-                                          const uint32_t *const *init_bits, hipStream_t s);
-extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
-                                            uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, hipStream_t s);
+#include "pt_host.hpp"
 
 namespace tamc {
-
-constexpr uint32_t kLdsLimit = 160u * 1024u;
-
-template <class T>
-static T *host_dup(const std::vector<T> &v) {
-  T *p = (T *)malloc(std::max<size_t>(1, v.size() * sizeof(T)));
-  if (!v.empty()) memcpy(p, v.data(), v.size() * sizeof(T));
-  return p;
-}
-
-struct GroupPlan {
-  uint32_t chain0, nch, R, G, P;
-  bool icm_global = false;
-  PtGroup g{};  // device pointers filled at allocation
-  std::vector<uint32_t> init_bits;
-};
-
-struct SolvePlan {
-  const tamc_instance *inst = nullptr;
-  tamc_pt_params P{};
-  std::vector<float> betas;
-  const int8_t *init_states = nullptr;
-  uint32_t n = 0, B = 0, C = 0, T = 0, meas_init = 0, n_meas = 0, n_samp_events = 0, n_samp_slots = 0, nbytes = 0,
-           gs_cap = 0, gs_words = 0, npairs = 0, nchi = 0;
-  std::vector<GroupPlan> groups;
-  float *d_betas = nullptr;
-};
 
 static uint32_t count_events(uint32_t T, uint32_t meas_init, uint32_t S) {
   if (T <= meas_init || S == 0) return 0;
@@ -55,7 +22,7 @@ static uint32_t count_events(uint32_t T, uint32_t meas_init, uint32_t S) {
   return last / S + 1 + (last % S ? 1 : 0);
 }
 
-static tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_params *P,
+tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_params *P,
                               const int8_t *init_states, uint32_t waves) {
   if (!inst || !P) return fail(TAMC_E_INVALID_ARG, "null argument");
   if (P->num_betas < 1 || !P->betas) return fail(TAMC_E_INVALID_ARG, "num_betas must be >= 1");
@@ -118,99 +85,108 @@ static tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tam
   return TAMC_OK;
 }
 
-static uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
+uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
   return with_nbr(make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global), S.n,
                   g.g.ell_col16 ? g.g.ell_D : 0u, kLdsLimit)
       .total;
 }
 
-// Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
-static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
-  for (auto &S : solves) {
-    A.want(&S.d_betas, S.B);
-    for (auto &g : S.groups) {
-      PtGroup &X = g.g;
-      const uint32_t nb = (S.n + 63) / 64;
-      A.want(&X.spins, (size_t)S.n * g.G * 2);
-      A.want(&X.energy, g.R);
-      A.want(&X.drift, g.R);
-      A.want(&X.phase_ticks, 6);
-      A.want(&X.perm, (size_t)g.nch * S.B);
-      A.want(&X.tags, (size_t)g.nch * S.B);
-      A.want(&X.nacc, (size_t)g.nch * S.B);
-      A.want(&X.hist, (size_t)g.nch * S.B * 2);
-      A.want(&X.rt, g.nch);
-      A.want(&X.gs_min, 1);
-      A.want(&X.gs_count, 1);
-      A.want(&X.icm_moves, 1);
-      A.want(&X.out_e, (size_t)S.n_meas * g.nch * S.B);
-      A.want(&X.out_q, (size_t)S.n_meas * S.B);
-      A.want(&X.out_chi, (size_t)S.n_meas * S.B * S.nchi);
-      A.want(&X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes);
-      A.want(&X.out_gs_steps, S.gs_cap);
-      A.want(&X.out_gs_e, S.gs_cap);
-      A.want(&X.out_gs_states, (size_t)S.gs_cap * S.gs_words);
-      if (g.icm_global) {
-        A.want(&X.g_qw, S.n);
-        A.want(&X.g_cw, S.n);
-        A.want(&X.g_blk, (size_t)nb * g.P);
-      }
+void want_buffers(SolvePlan &S, Arena &A) {
+  A.want(&S.d_betas, S.B);
+  for (auto &g : S.groups) {
+    PtGroup &X = g.g;
+    const uint32_t nb = (S.n + 63) / 64;
+    A.want(&X.spins, (size_t)S.n * g.G * 2);
+    A.want(&X.energy, g.R);
+    A.want(&X.drift, g.R);
+    A.want(&X.phase_ticks, 6);
+    A.want(&X.perm, (size_t)g.nch * S.B);
+    A.want(&X.tags, (size_t)g.nch * S.B);
+    A.want(&X.nacc, (size_t)g.nch * S.B);
+    A.want(&X.hist, (size_t)g.nch * S.B * 2);
+    A.want(&X.rt, g.nch);
+    A.want(&X.gs_min, 1);
+    A.want(&X.gs_count, 1);
+    A.want(&X.icm_moves, 1);
+    A.want(&X.out_e, (size_t)S.n_meas * g.nch * S.B);
+    A.want(&X.out_q, (size_t)S.n_meas * S.B);
+    A.want(&X.out_chi, (size_t)S.n_meas * S.B * S.nchi);
+    A.want(&X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes);
+    A.want(&X.out_gs_steps, S.gs_cap);
+    A.want(&X.out_gs_e, S.gs_cap);
+    A.want(&X.out_gs_states, (size_t)S.gs_cap * S.gs_words);
+    if (g.icm_global) {
+      A.want(&X.g_qw, S.n);
+      A.want(&X.g_cw, S.n);
+      A.want(&X.g_blk, (size_t)nb * g.P);
     }
   }
+}
+
+tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
+  if (s) TAMC_HIP(hipMemcpyAsync(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice, s));
+  else TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
+  for (auto &g : S.groups) {
+    tamc_status st;
+    DeviceInstance *D = device_instance(S.inst, g.G, &st);
+    if (!D) return st;
+    const DeviceSchedule *Sc = device_schedule(S.inst, g.G, &st);
+    if (!Sc) return st;
+    PtGroup &X = g.g;
+    X.row_ptr = (const uint32_t *)D->row_ptr.p;
+    X.col = (const uint32_t *)D->col.p;
+    X.J = (const float *)D->J.p;
+    X.bias = (const float *)D->bias.p;
+    X.g_row_ptr = (const uint32_t *)D->g_row_ptr.p;
+    X.g_col = (const uint32_t *)D->g_col.p;
+    X.chi = (const double *)D->chi.p;
+    X.ell_col = (const uint32_t *)D->ell_col.p;
+    X.ell_col16 = S.n < 65536u ? (const uint16_t *)D->ell_col16.p : nullptr;
+    X.ell_J = (const float *)D->ell_J.p;
+    X.ell_D = D->ell_D;
+    X.self_J = D->has_self ? (const float *)D->self_J.p : nullptr;
+    X.offset = S.inst->offset;
+    X.n = S.n;
+    X.g_nodes = S.inst->g_nodes;
+    X.nchi = S.nchi;
+    X.nblk = (S.n + 63) / 64;
+    X.lvl_off = (const uint32_t *)Sc->lvl_off.p;
+    X.tasks = (const uint32_t *)Sc->tasks.p;
+    X.n_levels = Sc->n_levels;
+    X.grp = (const uint32_t *)Sc->groups.p;
+    X.betas = S.d_betas;
+    X.B = S.B;
+    X.nch = g.nch;
+    X.chain0 = g.chain0;
+    X.lo = S.P.lo_beta_idx;
+    X.G = g.G;
+    X.R = g.R;
+    X.P = g.P;
+    X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
+              (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
+              (g.icm_global ? kFlagIcmGlobal : 0u);
+    X.T = S.T;
+    X.meas_init = S.meas_init;
+    X.sample = S.P.sample_enabled ? S.P.sample : 1u;
+    X.sample_states = S.P.sample_states_enabled ? S.P.sample_states : 1u;
+    X.sample_limiting = S.P.sample_limiting;
+    X.n_samp_slots = S.n_samp_slots;
+    X.gs_cap = S.gs_cap;
+    X.nbytes = S.nbytes;
+    X.gs_words = S.gs_words;
+    X.pair_global = g.chain0 / 2;
+    X.seed = S.P.seed;
+  }
+  return TAMC_OK;
+}
+
+// Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
+static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
+  for (auto &S : solves) want_buffers(S, A);
   TAMC_HIP(A.commit());
   for (auto &S : solves) {
-    TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
-    for (auto &g : S.groups) {
-      tamc_status st;
-      DeviceInstance *D = device_instance(S.inst, g.G, &st);
-      if (!D) return st;
-      const DeviceSchedule *Sc = device_schedule(S.inst, g.G, &st);
-      if (!Sc) return st;
-      PtGroup &X = g.g;
-      X.row_ptr = (const uint32_t *)D->row_ptr.p;
-      X.col = (const uint32_t *)D->col.p;
-      X.J = (const float *)D->J.p;
-      X.bias = (const float *)D->bias.p;
-      X.g_row_ptr = (const uint32_t *)D->g_row_ptr.p;
-      X.g_col = (const uint32_t *)D->g_col.p;
-      X.chi = (const double *)D->chi.p;
-      X.ell_col = (const uint32_t *)D->ell_col.p;
-      X.ell_col16 = S.n < 65536u ? (const uint16_t *)D->ell_col16.p : nullptr;
-      X.ell_J = (const float *)D->ell_J.p;
-      X.ell_D = D->ell_D;
-      X.self_J = D->has_self ? (const float *)D->self_J.p : nullptr;
-      X.offset = S.inst->offset;
-      X.n = S.n;
-      X.g_nodes = S.inst->g_nodes;
-      X.nchi = S.nchi;
-      X.nblk = (S.n + 63) / 64;
-      X.lvl_off = (const uint32_t *)Sc->lvl_off.p;
-      X.tasks = (const uint32_t *)Sc->tasks.p;
-      X.n_levels = Sc->n_levels;
-      X.grp = (const uint32_t *)Sc->groups.p;
-      X.betas = S.d_betas;
-      X.B = S.B;
-      X.nch = g.nch;
-      X.chain0 = g.chain0;
-      X.lo = S.P.lo_beta_idx;
-      X.G = g.G;
-      X.R = g.R;
-      X.P = g.P;
-      X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
-                (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
-                (g.icm_global ? kFlagIcmGlobal : 0u);
-      X.T = S.T;
-      X.meas_init = S.meas_init;
-      X.sample = S.P.sample_enabled ? S.P.sample : 1u;
-      X.sample_states = S.P.sample_states_enabled ? S.P.sample_states : 1u;
-      X.sample_limiting = S.P.sample_limiting;
-      X.n_samp_slots = S.n_samp_slots;
-      X.gs_cap = S.gs_cap;
-      X.nbytes = S.nbytes;
-      X.gs_words = S.gs_words;
-      X.pair_global = g.chain0 / 2;
-      X.seed = S.P.seed;
-    }
+    const tamc_status st = fill_groups(S, nullptr);
+    if (st) return st;
   }
   return TAMC_OK;
 }
@@ -273,8 +249,7 @@ static tamc_status d2h(std::vector<T> &v, const T *d, size_t count) {
   return TAMC_OK;
 }
 
-// Gather one solve's group outputs into a tamc_pt_result laid out like the reference structures.
-static tamc_status assemble(const SolvePlan &S, tamc_pt_result **out) {
+tamc_status assemble(const SolvePlan &S, tamc_pt_result **out) {
   const uint32_t n = S.n, B = S.B, C = S.C, nm = S.n_meas, np = S.npairs;
   tamc_pt_result *R = (tamc_pt_result *)calloc(1, sizeof(tamc_pt_result));
   R->n = n;
@@ -462,13 +437,13 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (opt.use_graph) {
     TAMC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, s);
+      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s);
     TAMC_HIP(hipStreamEndCapture(s, &graph));
     TAMC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     TAMC_HIP(hipGraphLaunch(gexec, s));
   } else {
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, s));
+      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s));
   }
   TAMC_HIP(hipEventRecord(e1, s));
   TAMC_HIP(hipEventSynchronize(e1));
@@ -515,7 +490,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   return TAMC_OK;
 }
 
-static uint32_t default_waves(const tamc_instance *inst) {
+uint32_t default_waves(const tamc_instance *inst) {
   // enough waves to cover the typical level width, within one work group
   const uint32_t nq = inst->n_groups;
   const uint32_t width = inst->n_levels ? (nq + inst->n_levels - 1) / inst->n_levels : 1;

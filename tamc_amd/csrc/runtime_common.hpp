@@ -57,17 +57,25 @@ struct Arena {
     reqs.emplace_back((void **)slot, count * sizeof(T));
   }
   DevBuf buf;
-  hipError_t commit() {
-    size_t total = 0;
-    for (auto &r : reqs) total += (r.second + 255) & ~(size_t)255;
-    hipError_t e = buf.alloc(total);
-    if (e != hipSuccess) return e;
+  // bytes the registered buffers take, each rounded up to 256
+  size_t total() const {
+    size_t t = 0;
+    for (auto &r : reqs) t += (r.second + 255) & ~(size_t)255;
+    return t;
+  }
+  // point every registered slot into [base, base + total()) (base 256-aligned, owned elsewhere)
+  void place(void *base) {
     size_t off = 0;
     for (auto &r : reqs) {
-      *r.first = (char *)buf.p + off;
+      *r.first = (char *)base + off;
       off += (r.second + 255) & ~(size_t)255;
     }
-    used = total;
+    used = off;
+  }
+  hipError_t commit() {
+    hipError_t e = buf.alloc(total());
+    if (e != hipSuccess) return e;
+    place(buf.p);
     return hipSuccess;
   }
 };
