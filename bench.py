@@ -70,6 +70,47 @@ def cpu_baseline(lines, params, sweeps, threads):
     return attempts / dt, attempts, dt
 
 
+def serving_leg(n_req, rate, sweeps, slice_ms):
+    """Serving workload: a seeded Poisson stream of mixed PT-ICM requests (2D L 8..128, 3D L 4..24,
+    random 3-regular N 64..16384; +-J or N(0,1); 32 betas 0.2-5.0, 2 chains, `sweeps` sweeps) fed to
+    the continuous-batching server at their arrival times.  Latency = submit -> result on the host."""
+    from tamc_amd import api, workloads
+    reqs = workloads.serving_requests(n_req, rate, seed=105)
+    insts = [api.BqmIsingInstance.from_lines(i, j, K) for (_, _, _, (i, j, K, n)) in reqs]
+
+    def params(k):
+        return api.PtIcmParams(num_sweeps=sweeps, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 32),
+                               lo_beta=1.0, icm=True, num_replica_chains=2, sample=32, sample_states=32,
+                               sample_limiting=2, seed=5000 + k)
+
+    with api.Server(slice_ms=slice_ms) as srv:
+        srv.wait(srv.submit(insts[0], params(0)))  # warm-up request
+        tickets = []
+        t0 = time.perf_counter()
+        for k, (arrival, _, _, _) in enumerate(reqs):
+            delay = arrival - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(insts[k], params(k)))
+        submitted_s = time.perf_counter() - t0
+        lat, attempts = [], 0
+        for t in tickets:
+            res, tm = srv.wait(t)
+            lat.append(tm["total_ms"])
+            attempts += res["attempts"]
+        makespan = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array(lat)
+    return {
+        "requests": n_req, "arrival_rate_per_s": rate, "sweeps": sweeps, "slice_ms": slice_ms,
+        "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
+        "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "throughput_attempts_per_s": attempts / makespan, "makespan_s": makespan, "arrivals_s": submitted_s,
+        "offered_attempts_per_s": attempts / reqs[-1][0], "slices": st["slices"],
+        "device_busy_s": st["device_busy_ms"] / 1e3,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +126,11 @@ def main():
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-sweeps", type=int, default=2000)
+    ap.add_argument("--serve-slice-ms", type=float, default=4.0)
     ap.add_argument("--only-largest", action="store_true", help="run only the largest instance (profiling)")
     ap.add_argument("--no-icm", action="store_true", help="switch ICM off (profiling the sweep alone)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -149,7 +195,7 @@ def main():
     default_workload = (args.instances == 256 and args.sweeps == 4000 and not args.only_largest
                         and not args.no_icm)
     out = {
-        "metric": "spin-flip attempts/s per GPU (batched PT-ICM)",
+        "metric": "spin-flip attempts/s per GPU (batched PT-ICM vs plain) + p50/p99 solve latency",
         "value": value,
         "unit": "spin-flip attempts/s",
         "n_gpus": 1,
@@ -179,6 +225,8 @@ def main():
                      "algorithmic_bytes_per_launch": attempts_per_step * BYTES_PER_ATTEMPT_D3,
                      "kernel_ms": float(np.mean(dev_ms))},
     }
+    if not args.no_serving:
+        out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_sweeps, args.serve_slice_ms)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rate, att, dt = cpu_baseline(lines[:args.cpu_instances], params[:args.cpu_instances], args.cpu_sweeps,

@@ -532,6 +532,11 @@ tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, cons
   if (r->S.groups.size() > srv->max_groups)
     return fail(TAMC_E_UNSUPPORTED, "request has more chain pairs than the server's work-group slots");
   r->S.init_states = nullptr;  // copied into the groups' init_bits
+  // the instance is uploaded (once per instance) here, in the caller's thread, not in the loop
+  for (auto &g : r->S.groups) {
+    if (!device_instance(inst, g.G, &s)) return s;
+    if (!device_schedule(inst, g.G, &s)) return s;
+  }
   for (auto &g : r->S.groups) r->lds = std::max(r->lds, lds_bytes_for(r->S, g, r->waves));
   r->lds_class = tamc_server::lds_class_of(r->lds);
   std::lock_guard<std::mutex> lk(srv->mu);

@@ -109,3 +109,31 @@ def config3_instances(count=256, seed=103, nmin=200, nmax=4000):
         n = int(rng.integers(nmin // 2, nmax // 2 + 1)) * 2
         out.append(random_regular3(n, rng, gaussian=True, bias_sigma=0.1))
     return out
+
+
+def serving_requests(count=2000, rate=200.0, seed=105):
+    """Serving workload: Poisson arrivals at `rate` per second; each request is, with equal
+    probability, a periodic 2D lattice L in [8,128], a periodic 3D lattice L in [4,24] or a random
+    3-regular graph N in [64,16384] (even), with +-J or N(0,1) couplings (probability 1/2 each).
+    Returns a list of (arrival_s, kind, size, (i, j, K, n))."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += rng.exponential(1.0 / rate)
+        kind = int(rng.integers(0, 3))
+        pm = bool(rng.integers(0, 2))
+        sub = np.random.default_rng(int(rng.integers(0, 2**63 - 1)))
+        if kind == 0:
+            L = int(rng.integers(8, 129))
+            lines = lattice_2d(L, sub, pm_j=pm)
+            out.append((t, "2d", L, lines))
+        elif kind == 1:
+            L = int(rng.integers(4, 25))
+            lines = lattice_3d(L, sub, pm_j=pm)
+            out.append((t, "3d", L, lines))
+        else:
+            n = int(rng.integers(32, 8193)) * 2
+            lines = random_regular3(n, sub, gaussian=not pm)
+            out.append((t, "rr3", n, lines))
+    return out

@@ -105,6 +105,16 @@ def test_pt_production_gaussian(gpu, kind, kw):
     assert_same(g, o, exact_energy=False)
 
 
+@pytest.mark.parametrize("kind,kw,exact", [("2d_pm", dict(L=128), True), ("3d_pm", dict(L=24), True),
+                                           ("rr3", dict(n=16384), False)])
+def test_pt_production_large_instances(gpu, kind, kw, exact):
+    # N up to 16384 with 32 betas x 2 chains: the spins still fit in LDS, the ICM cluster words and
+    # the neighbour table do not, so ICM runs on its global-memory scratch
+    g, o = _run_both(kind, kw, B=32, lo=16, T=6, sample=2, sstates=3, limiting=2)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=exact)
+
+
 def test_pt_large_ladder_two_words(gpu):
     # 48 betas x 2 chains = 96 replicas: two bit-plane words per site
     g, o = _run_both("3d_pm", dict(L=3), B=48, lo=28, T=24, sample=3, sstates=5)
