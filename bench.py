@@ -166,7 +166,7 @@ def main():
 
     def step():
         return api.run_pt_batch(insts, params, waves_per_group=args.waves, sweeps_per_launch=args.sweeps_per_launch,
-                                use_graph=args.graph)
+                                use_graph=args.graph, autotune=True)
 
     for _ in range(args.warmup):
         step()

@@ -156,10 +156,12 @@ void tamc_pt_result_free(tamc_pt_result *r);
 
 /* ------------------------------------------------------------ batched PT ---- */
 typedef struct {
-  uint32_t waves_per_group;  /* 0 = autotune */
-  uint32_t sweeps_per_launch;/* 0 = autotune (all sweeps in one launch when 0 and not autotuning) */
+  uint32_t waves_per_group;  /* 0 = chosen by the autotuner when `autotune` is set, else from the
+                                instances' level widths */
+  uint32_t sweeps_per_launch;/* 0 = all sweeps in one launch */
   int32_t use_graph;         /* capture the launch sequence in a hipGraph */
-  int32_t autotune;          /* time candidate tilings on a short prefix and keep the fastest */
+  int32_t autotune;          /* time 4, 8 and 16 waves per group on a 24-sweep prefix of the same batch
+                                and keep the fastest (cached per batch shape) */
 } tamc_batch_options;
 
 typedef struct {
@@ -167,8 +169,9 @@ typedef struct {
   double wall_ms;
   uint64_t attempts;
   uint32_t waves_per_group, sweeps_per_launch, launches, groups;
-  double phase_ms[6];        /* slowest work group: ICM overlap+seeds, ICM cluster growth, ICM flip,
-                                exchange, sweep, measurement time */
+  double phase_ms[7];        /* slowest work group: ICM overlap words, ICM seeds, ICM cluster growth,
+                                ICM flip, exchange, sweep, measurement time */
+  double icm_passes;         /* slowest work group: ICM cluster-growth passes, summed over sweeps */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

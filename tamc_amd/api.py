@@ -262,7 +262,8 @@ def _batch_args(instances, params_list):
     return resolved, cps, hs
 
 
-def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False):
+def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False,
+                 autotune=False):
     """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path."""
     resolved, cps, hs = _batch_args(instances, params_list)
     outs = (C.POINTER(PtResult) * len(instances))()
@@ -270,7 +271,7 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
     if plain:
         check(lib().tamc_pt_run_plain(len(instances), hs, cps, outs, C.byref(stats)))
     else:
-        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), 0)
+        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune))
         check(lib().tamc_pt_run_batch(len(instances), hs, cps, C.byref(opt), outs, C.byref(stats)))
     res = []
     try:
@@ -281,7 +282,8 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
             lib().tamc_pt_result_free(outs[k])
     st = dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
-              launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms))
+              launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
+              icm_passes=stats.icm_passes)
     return res, st
 
 

@@ -51,13 +51,20 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
                        uint32_t waves);
 // Dynamic LDS bytes the sweep kernel needs for group g at `waves` waves.
 uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves);
-// Register every device buffer of S in the arena.
-void want_buffers(SolvePlan &S, Arena &A);
+// Register every device buffer of S in the arena (the ground-state trace in Ags when given).
+void want_buffers(SolvePlan &S, Arena &A, Arena *Ags = nullptr);
 // After the arena is placed: upload the betas (on stream s, or synchronously when s is null) and
 // fill the groups' PtGroup descriptors.
 tamc_status fill_groups(SolvePlan &S, hipStream_t s);
-// Gather one solve's group outputs into a tamc_pt_result laid out like the reference structures.
-tamc_status assemble(const SolvePlan &S, tamc_pt_result **out);
+// Host copy of a device range: result arrays inside it are read from `host`.
+struct HostMirror {
+  const char *host;
+  const char *dev;
+  size_t bytes;
+};
+// Gather one solve's group outputs into a tamc_pt_result laid out like the reference structures
+// (m: optional host mirror of the arena holding them).
+tamc_status assemble(const SolvePlan &S, tamc_pt_result **out, const HostMirror *m = nullptr);
 // Waves per work group covering the instance's typical level width.
 uint32_t default_waves(const tamc_instance *inst);
 

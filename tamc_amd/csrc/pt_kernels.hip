@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "pt_types.hpp"
 #include "tamc_math.hpp"
@@ -29,6 +30,8 @@ namespace tamc {
 
 // Read-only device data is accessed through the constant address space: wave-uniform indices then
 // become scalar (SMEM) loads, divergent ones ordinary global loads.
+#define TAMC_AS1 __attribute__((address_space(1)))
+#define TAMC_AS3 __attribute__((address_space(3)))
 #define TAMC_AS4 __attribute__((address_space(4)))
 template <class T>
 __device__ __forceinline__ const TAMC_AS4 T *c4(const T *p) {
@@ -121,7 +124,7 @@ __global__ void pt_init_kernel(const PtGroup *groups, const uint32_t *const *ini
     X.energy[r] = total;
     X.drift[r] = 0;
   }
-  if (threadIdx.x < 6 && X.phase_ticks) X.phase_ticks[threadIdx.x] = 0ull;
+  if (threadIdx.x < kPhaseSlots && X.phase_ticks) X.phase_ticks[threadIdx.x] = 0ull;
   for (uint32_t k = threadIdx.x; k < X.nch * B; k += blockDim.x) {
     const uint32_t b = k % B;
     X.perm[k] = k;  // chain c slot b holds replica c*B+b
@@ -144,11 +147,13 @@ struct Ctx {
   uint32_t *sp;      // LDS bit planes
   float *energy, *beta_of, *dEx;
   long long *partial, *drift;
-  uint32_t *perm, *pair_a, *pair_b, *pair_of, *seed_blk, *seed_rank, *misc;
+  uint32_t *perm, *pair_a, *pair_b, *seed_blk, *seed_rank, *misc;
+  uint32_t *moves;   // LDS count of ICM moves in this launch
   uint8_t *tags;
   uint32_t *nacc, *hist, *rt;
   uint32_t *qw, *cw, *blk, *uw;
   const uint16_t *nbr;  // ICM neighbour lists [n][D] (LDS copy or global)
+  bool nbr_in_lds;
   long long *dEa;
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
 };
@@ -172,130 +177,123 @@ struct SiteWords {
   }
 };
 
-// Wave-wide sums and ORs with DPP row shifts and row broadcasts (an inclusive scan whose last lane
-// holds the total); every lane of the wave must be active.
-#define TAMC_DPP_STEP64(ctrl, rmask)                                                   \
-  {                                                                                    \
-    const uint32_t l2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rmask, 0xf, false); \
-    const uint32_t h2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rmask, 0xf, false); \
-    const uint64_t s = (((uint64_t)hi << 32) | lo) + (((uint64_t)h2 << 32) | l2);      \
-    lo = (uint32_t)s;                                                                  \
-    hi = (uint32_t)(s >> 32);                                                          \
-  }
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-  TAMC_DPP_STEP64(0x111, 0xf)  // row_shr:1
-  TAMC_DPP_STEP64(0x112, 0xf)  // row_shr:2
-  TAMC_DPP_STEP64(0x114, 0xf)  // row_shr:4
-  TAMC_DPP_STEP64(0x118, 0xf)  // row_shr:8
-  TAMC_DPP_STEP64(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
-  TAMC_DPP_STEP64(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
-  const uint32_t tl = __builtin_amdgcn_readlane(lo, 63), th = __builtin_amdgcn_readlane(hi, 63);
-  return ((uint64_t)th << 32) | tl;
+// OR over the lanes of each group of GS consecutive lanes (GS = 16 or 32) with DPP row shifts
+// (and a row broadcast for 32); the result is complete in the group's last lane.  Every lane of
+// the wave must be active.
+template <int GS>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  if constexpr (GS == 32) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+  return v;
 }
-#undef TAMC_DPP_STEP64
-#define TAMC_DPP_OR(ctrl, rmask) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, rmask, 0xf, false);
-__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
-  TAMC_DPP_OR(0x111, 0xf)
-  TAMC_DPP_OR(0x112, 0xf)
-  TAMC_DPP_OR(0x114, 0xf)
-  TAMC_DPP_OR(0x118, 0xf)
-  TAMC_DPP_OR(0x142, 0xa)
-  TAMC_DPP_OR(0x143, 0xc)
-  return __builtin_amdgcn_readlane(v, 63);
-}
-#undef TAMC_DPP_OR
 
-// ICM step (d) over sites, lane = site.  For fixed-stride rows (D > 0) the site's couplings and its
-// neighbours' cluster words are loaded once and reused for every pair the site belongs to.  Pair p only reads neighbours outside its cluster and only flips
-// replicas a_p, b_p of sites inside it, so waves may flip their blocks as soon as they are done.
-template <int D>
-__device__ __forceinline__ void icm_flip(Ctx &c) {
+// ICM step (d): swap the cluster spins of both replicas of every pair (= flip both).  Lanes are
+// (site slot, pair): a wave takes 64/GS sites per step and lane (s, p) handles pair p at site i_s.
+// When i_s is in cluster p the lane adds the first replica's energy change
+//   -2 s_a(i) (h_i + sum_{j not in cluster p} J_ij s_a(j))
+// to its own fixed-point accumulator; the second replica's change is its negative (the move
+// conserves E_a + E_b).  The flip masks of a site are ORed over its group of lanes and written by
+// the group's last lane.  Pair p only reads replica a_p at neighbours outside cluster p and only
+// flips replicas a_p, b_p at sites inside it, so sites are flipped as soon as they are done.
+// ICM scratch -- overlap words qw, cluster words cw, per-block counts blk -- and the u16
+// neighbour lists, with address spaces fixed at compile time: the scratch is in LDS (kLds) or in
+// HBM for groups too large for LDS, and the neighbour lists are an LDS copy or the HBM table.
+// Typed pointers make every access a ds_* or global_* instruction instead of a flat one (flat
+// accesses wait on both the LDS and the vector-memory counters).
+template <bool kLds>
+struct IcmMem {
+  using U32 = std::conditional_t<kLds, TAMC_AS3 uint32_t, TAMC_AS1 uint32_t>;
+  U32 *qw, *cw, *blk;
+  const TAMC_AS3 uint16_t *nbr_lds;  // null when the lists are not in LDS
+  const TAMC_AS4 uint16_t *nbr_glb;  // null when the instance has no fixed-stride lists
+  __device__ __forceinline__ uint32_t nbr(size_t k) const { return nbr_lds ? nbr_lds[k] : nbr_glb[k]; }
+};
+
+template <int D, int GS, bool kLds>
+__device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t P = c.P, n = c.n, G = c.G;
-  constexpr int DD = D > 0 ? D : 1;
-  // replica indices of every pair, lane p holding pair p
-  const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
-  const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
-  for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
-    const uint32_t i = blk * 64u + c.lane;
-    const uint32_t cwi = (i < n) ? c.cw[i] : 0u;
-    uint32_t act = wave_or_u32(cwi);
-    if (act == 0u) continue;
-    SiteWords h{0ull, c.sp, 0u, G};
-    unsigned long long M[4] = {0, 0, 0, 0};
-    float bias_i = 0.f;
-    uint32_t nj[DD], ncw[DD];
-    float nJ[DD];
-#pragma unroll
-    for (int k = 0; k < DD; ++k) {
-      nj[k] = 0u;
-      ncw[k] = ~0u;
-      nJ[k] = 0.f;
-    }
-    if (cwi) {
-      h.load(c.sp, i, G);
-      bias_i = c4(X.bias)[i];
+  constexpr uint32_t SPW = 64u / GS;
+  const uint32_t p = c.lane % GS, slot = c.lane / GS;
+  const bool pv = p < P;
+  const uint32_t a = pv ? c.pair_a[p] : 0u, b = pv ? c.pair_b[p] : 0u;
+  long long acc = 0;
+  const uint32_t steps = (n + SPW - 1u) / SPW;
+  for (uint32_t st = c.wave; st < steps; st += c.W) {
+    const uint32_t i = st * SPW + slot;
+    const uint32_t cwi = (i < n) ? M.cw[i] : 0u;
+    if (ballot(cwi != 0u) == 0ull) continue;
+    const bool in = pv && ((cwi >> p) & 1u);
+    if (in) {
+      float hh = c4(X.bias)[i];
       if constexpr (D > 0) {
+        uint32_t nj[D];
+        float nJ[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          nj[k] = c4(X.ell_col)[(size_t)i * D + k];
+          nj[k] = M.nbr((size_t)i * D + k);
           nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
         }
+        uint32_t out[D], sj[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) ncw[k] = c.cw[nj[k]];
-
-      }
-    }
-    while (act) {
-      const uint32_t p = (uint32_t)__builtin_ctz(act);
-      act &= act - 1u;
-      const uint32_t a = __builtin_amdgcn_readlane(pa_l, p), b = __builtin_amdgcn_readlane(pb_l, p);
-      const uint32_t bp = (cwi >> p) & 1u;
-      long long v = 0;
-      if (bp) {
-        float hh = bias_i;
-        if constexpr (D > 0) {
-          uint32_t sj[D];
+        for (int k = 0; k < D; ++k) {
+          out[k] = ((M.cw[nj[k]] >> p) & 1u) ^ 1u;
+          sj[k] = sbit(c.sp, G, nj[k], a);
+        }
 #pragma unroll
-            for (int k = 0; k < D; ++k) sj[k] = sbit(c.sp, G, nj[k], a);
-#pragma unroll
-          for (int k = 0; k < D; ++k)
-            if (!((ncw[k] >> p) & 1u)) hh += sj[k] ? -nJ[k] : nJ[k];
-        } else {
-          const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
-          for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t j = c4(X.col)[e];
-            if (!((c.cw[j] >> p) & 1u)) {
-              const float Je = c4(X.J)[e];
-              hh += sbit(c.sp, G, j, a) ? -Je : Je;
-            }
+        for (int k = 0; k < D; ++k)
+          if (out[k]) hh += sj[k] ? -nJ[k] : nJ[k];
+      } else {
+        const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
+        for (uint32_t e = e0; e < e1; ++e) {
+          const uint32_t j = c4(X.col)[e];
+          if (!((M.cw[j] >> p) & 1u)) {
+            const float Je = c4(X.J)[e];
+            hh += sbit(c.sp, G, j, a) ? -Je : Je;
           }
         }
-        const float term = h.bit(a) ? 2.0f * hh : -2.0f * hh;
-        v = to_fix(term);
-#pragma unroll
-        for (uint32_t g = 0; g < 4u; ++g) {
-          if ((a >> 6) == g) M[g] |= 1ull << (a & 63u);
-          if ((b >> 6) == g) M[g] |= 1ull << (b & 63u);
+      }
+      const float term = sbit(c.sp, G, i, a) ? 2.0f * hh : -2.0f * hh;
+      acc += to_fix(term);
+    }
+    for (uint32_t g = 0; g < G; ++g) {
+      uint32_t lo = 0u, hi = 0u;
+      if (in) {
+        if ((a >> 6) == g) {
+          if (a & 32u) hi |= 1u << (a & 31u);
+          else lo |= 1u << (a & 31u);
+        }
+        if ((b >> 6) == g) {
+          if (b & 32u) hi |= 1u << (b & 31u);
+          else lo |= 1u << (b & 31u);
         }
       }
-      const uint64_t tot = wave_sum_u64((uint64_t)v);
-      if (c.lane == 0u) atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)tot);
-    }
-#pragma unroll
-    for (uint32_t g = 0; g < 4u; ++g)
-      if (g < G && M[g]) {
-        c.sp[(i * G + g) * 2u] ^= (uint32_t)M[g];
-        c.sp[(i * G + g) * 2u + 1u] ^= (uint32_t)(M[g] >> 32);
+      lo = group_or<GS>(lo);
+      hi = group_or<GS>(hi);
+      if (p == GS - 1u && i < n) {
+        if (lo) c.sp[(i * G + g) * 2u] ^= lo;
+        if (hi) c.sp[(i * G + g) * 2u + 1u] ^= hi;
       }
+    }
   }
+  if (pv && acc != 0) atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)acc);
 }
 
 // 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
 // the two chains; bit p of the per-site words qw/cw belongs to pair p.
+template <bool kLds>
 __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long *ticks, unsigned long long &tk) {
   const TAMC_AS4 PtGroup &X = *c.X;
+  using U32 = typename IcmMem<kLds>::U32;
+  IcmMem<kLds> M;
+  M.qw = (U32 *)c.qw;
+  M.cw = (U32 *)c.cw;
+  M.blk = (U32 *)c.blk;
+  M.nbr_lds = c.nbr_in_lds ? (const TAMC_AS3 uint16_t *)c.nbr : nullptr;
+  M.nbr_glb = c4(X.ell_col16);
   const uint32_t P = c.P, n = c.n, G = c.G, D = c.nbr ? X.ell_D : 0u;
   if (c.tid < P) {
     const uint32_t b = c.lo + c.tid;
@@ -303,12 +301,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     c.pair_b[c.tid] = c.perm[c.B + b];
     c.dEa[c.tid] = 0;
   }
-  for (uint32_t r = c.tid; r < c.R; r += c.nthreads) c.pair_of[r] = kNone;
   __syncthreads();
-  if (c.tid < P) {
-    c.pair_of[c.pair_a[c.tid]] = c.tid;
-    c.pair_of[c.pair_b[c.tid]] = c.tid;
-  }
   // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; per-block counts
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
     const uint32_t i = blk * 64u + c.lane;
@@ -327,21 +320,26 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
         const uint32_t qb = valid ? (h.bit(av[s]) ^ h.bit(bv[s])) : 0u;
         q |= qb << p;
         const uint32_t cnt = popc64(ballot(qb != 0u));
-        if (c.lane == 0) c.blk[blk * P + p] = cnt;
+        if (c.lane == 0) M.blk[blk * P + p] = cnt;
       }
     }
     if (valid) {
-      c.qw[i] = q;
-      c.cw[i] = 0u;
+      M.qw[i] = q;
+      M.cw[i] = 0u;
     }
   }
   __syncthreads();
+  {
+    const unsigned long long now = wall_clock64();
+    ticks[kPhIcmOverlap] += now - tk;
+    tk = now;
+  }
   // (b) seed choice: k = floor(w * count / 2^32) into the ascending list of q<0 sites
   if (c.wave == 0 && c.lane < P) {
     const uint32_t p = c.lane;
     uint32_t total = 0;
 #pragma unroll 8
-    for (uint32_t blk = 0; blk < c.nblk; ++blk) total += c.blk[blk * P + p];
+    for (uint32_t blk = 0; blk < c.nblk; ++blk) total += M.blk[blk * P + p];
     uint32_t sb = kNone, sr = 0;
     if (total) {
       uint32_t k;
@@ -354,7 +352,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
       }
       uint32_t cum = 0;
       for (uint32_t blk = 0; blk < c.nblk; ++blk) {
-        const uint32_t cb = c.blk[blk * P + p];
+        const uint32_t cb = M.blk[blk * P + p];
         if (k < cum + cb) {
           sb = blk;
           sr = k - cum;
@@ -371,35 +369,36 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     const uint32_t sb = c.seed_blk[p];
     if (sb == kNone) continue;
     const uint32_t i = sb * 64u + c.lane;
-    const uint32_t qb = (i < n) ? ((c.qw[i] >> p) & 1u) : 0u;
+    const uint32_t qb = (i < n) ? ((M.qw[i] >> p) & 1u) : 0u;
     const uint64_t m = ballot(qb != 0u);
-    if (qb && mbcnt64(m) == c.seed_rank[p]) atomicOr(&c.cw[i], 1u << p);
-    if (c.lane == 0) atomicAdd(X.icm_moves, 1ull);
+    if (qb && mbcnt64(m) == c.seed_rank[p]) atomicOr((uint32_t *)&M.cw[i], 1u << p);
+    if (c.lane == 0) atomicAdd(c.moves, 1u);
   }
   {
     const unsigned long long now = wall_clock64();
-    ticks[0] += now - tk;
+    ticks[kPhIcmSeed] += now - tk;
     tk = now;
   }
   // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
   //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing.
   for (;;) {
     __syncthreads();
+    ticks[kPhIcmPasses] += 1u;
     int changed = 0;
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-      const uint32_t q = c.qw[i];
+      const uint32_t q = M.qw[i];
       if (!q) continue;
-      const uint32_t cur = c.cw[i];
+      const uint32_t cur = M.cw[i];
       uint32_t nb = 0;
       if (D) {
-        for (uint32_t k = 0; k < D; ++k) nb |= c.cw[c.nbr[i * D + k]];
+        for (uint32_t k = 0; k < D; ++k) nb |= M.cw[M.nbr((size_t)i * D + k)];
       } else if (i < X.g_nodes) {
         const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
-        for (uint32_t e = e0; e < e1; ++e) nb |= c.cw[c4(X.g_col)[e]];
+        for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
       }
       const uint32_t nc = cur | (q & nb);
       if (nc != cur) {
-        c.cw[i] = nc;
+        M.cw[i] = nc;
         changed = 1;
       }
     }
@@ -407,15 +406,19 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
   }
   {
     const unsigned long long now = wall_clock64();
-    ticks[1] += now - tk;
+    ticks[kPhIcmGrow] += now - tk;
     tk = now;
   }
-  // (d) swap the cluster spins of both replicas (= flip both), lane = site.  The first replica's
-  //     energy change dE_a = sum_{i in cl} -2 s_a(i) (h_i + sum_{j notin cl} J_ij s_a(j)) is summed in
-  //     fixed point; the second replica's is -dE_a (the move conserves E_a + E_b).
-  if (X.ell_D == 4u) icm_flip<4>(c);
-  else if (X.ell_D == 8u) icm_flip<8>(c);
-  else icm_flip<0>(c);
+  // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
+  if (P <= 16u) {
+    if (X.ell_D == 4u) icm_flip<4, 16>(c, M);
+    else if (X.ell_D == 8u) icm_flip<8, 16>(c, M);
+    else icm_flip<0, 16>(c, M);
+  } else {
+    if (X.ell_D == 4u) icm_flip<4, 32>(c, M);
+    else if (X.ell_D == 8u) icm_flip<8, 32>(c, M);
+    else icm_flip<0, 32>(c, M);
+  }
   __syncthreads();
   // moved pairs: energies become energy_ref of the new states, as the reference recomputes them
   if (c.tid < P && c.seed_blk[c.tid] != kNone) {
@@ -769,7 +772,6 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.rt = (uint32_t *)(smem + L.rt);
   c.pair_a = (uint32_t *)(smem + L.pair_a);
   c.pair_b = (uint32_t *)(smem + L.pair_b);
-  c.pair_of = (uint32_t *)(smem + L.pair_of);
   c.seed_blk = (uint32_t *)(smem + L.seed_blk);
   c.seed_rank = (uint32_t *)(smem + L.seed_rank);
   c.misc = (uint32_t *)(smem + L.misc);
@@ -788,15 +790,19 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     c.blk = X.g_blk;
   }
   c.nbr = nullptr;
+  c.nbr_in_lds = false;
   if (c.P > 0 && X.ell_col16 != nullptr && X.ell_D != 0u) {
     if (L.nbr != kNone) {
       uint16_t *nb = (uint16_t *)(smem + L.nbr);
       for (uint32_t k = c.tid; k < c.n * X.ell_D; k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
       c.nbr = nb;
+      c.nbr_in_lds = true;
     } else {
       c.nbr = X.ell_col16;
     }
   }
+  c.moves = c.misc + 4;
+  if (c.tid == 0) c.moves[0] = 0u;
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
@@ -816,7 +822,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) c.beta_of[c.perm[k]] = c4(X.betas)[k % c.B];
   __syncthreads();
   const uint32_t tend = t1 < X.T ? t1 : X.T;
-  unsigned long long ticks[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long ticks[kPhaseSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
   unsigned long long tk = wall_clock64();
   auto lap = [&](int ph) {
     const unsigned long long now = wall_clock64();
@@ -824,18 +830,23 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     tk = now;
   };
   for (uint32_t t = t0; t < tend; ++t) {
-    if ((c.flags & kFlagIcm) && c.P > 0) icm_phase(c, t, ticks, tk);
-    lap(2);
+    if ((c.flags & kFlagIcm) && c.P > 0) {
+      if (use_lds_icm) icm_phase<true>(c, t, ticks, tk);
+      else icm_phase<false>(c, t, ticks, tk);
+    }
+    lap(kPhIcmFlip);
     if (c.flags & kFlagIcmOnly) break;
     exchange_phase(c, t);
-    lap(3);
+    lap(kPhExchange);
     sweep_phase(c, t);
-    lap(4);
+    lap(kPhSweep);
     measure_phase(c, t);
-    lap(5);
+    lap(kPhMeasure);
   }
   if (c.tid == 0 && X.phase_ticks)
-    for (int k = 0; k < 6; ++k) X.phase_ticks[k] += ticks[k];
+    for (int k = 0; k < (int)kPhaseSlots; ++k) X.phase_ticks[k] += ticks[k];
+  __syncthreads();
+  if (c.tid == 0) X.icm_moves[0] += c.moves[0];
   if (c.tid == 0 && launch_ticks) launch_ticks[blockIdx.x] = wall_clock64() - launch_start;
   // store persistent state
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];

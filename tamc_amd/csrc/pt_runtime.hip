@@ -9,6 +9,9 @@
 #include <chrono>
 #include <cmath>
 #include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -91,7 +94,7 @@ uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
       .total;
 }
 
-void want_buffers(SolvePlan &S, Arena &A) {
+void want_buffers(SolvePlan &S, Arena &A, Arena *Ags) {
   A.want(&S.d_betas, S.B);
   for (auto &g : S.groups) {
     PtGroup &X = g.g;
@@ -99,7 +102,7 @@ void want_buffers(SolvePlan &S, Arena &A) {
     A.want(&X.spins, (size_t)S.n * g.G * 2);
     A.want(&X.energy, g.R);
     A.want(&X.drift, g.R);
-    A.want(&X.phase_ticks, 6);
+    A.want(&X.phase_ticks, kPhaseSlots);
     A.want(&X.perm, (size_t)g.nch * S.B);
     A.want(&X.tags, (size_t)g.nch * S.B);
     A.want(&X.nacc, (size_t)g.nch * S.B);
@@ -114,7 +117,7 @@ void want_buffers(SolvePlan &S, Arena &A) {
     A.want(&X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes);
     A.want(&X.out_gs_steps, S.gs_cap);
     A.want(&X.out_gs_e, S.gs_cap);
-    A.want(&X.out_gs_states, (size_t)S.gs_cap * S.gs_words);
+    (Ags ? *Ags : A).want(&X.out_gs_states, (size_t)S.gs_cap * S.gs_words);
     if (g.icm_global) {
       A.want(&X.g_qw, S.n);
       A.want(&X.g_cw, S.n);
@@ -181,9 +184,10 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
 }
 
 // Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
-static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A) {
-  for (auto &S : solves) want_buffers(S, A);
+static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A, Arena &Ags) {
+  for (auto &S : solves) want_buffers(S, A, &Ags);
   TAMC_HIP(A.commit());
+  TAMC_HIP(Ags.commit());
   for (auto &S : solves) {
     const tamc_status st = fill_groups(S, nullptr);
     if (st) return st;
@@ -242,14 +246,22 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
   return TAMC_OK;
 }
 
+// Device -> host for result arrays: from the host mirror when the range lies inside it (one bulk
+// copy of the arena instead of one small copy per array), else a copy of its own.
 template <class T>
-static tamc_status d2h(std::vector<T> &v, const T *d, size_t count) {
+static tamc_status d2h(std::vector<T> &v, const T *d, size_t count, const HostMirror *m) {
   v.resize(count);
-  if (count) TAMC_HIP(hipMemcpy(v.data(), d, count * sizeof(T), hipMemcpyDeviceToHost));
+  if (!count) return TAMC_OK;
+  const char *dp = (const char *)d;
+  if (m && dp >= m->dev && dp + count * sizeof(T) <= m->dev + m->bytes) {
+    memcpy(v.data(), m->host + (dp - m->dev), count * sizeof(T));
+    return TAMC_OK;
+  }
+  TAMC_HIP(hipMemcpy(v.data(), d, count * sizeof(T), hipMemcpyDeviceToHost));
   return TAMC_OK;
 }
 
-tamc_status assemble(const SolvePlan &S, tamc_pt_result **out) {
+tamc_status assemble(const SolvePlan &S, tamc_pt_result **out, const HostMirror *m) {
   const uint32_t n = S.n, B = S.B, C = S.C, nm = S.n_meas, np = S.npairs;
   tamc_pt_result *R = (tamc_pt_result *)calloc(1, sizeof(tamc_pt_result));
   R->n = n;
@@ -289,20 +301,20 @@ tamc_status assemble(const SolvePlan &S, tamc_pt_result **out) {
     std::vector<uint32_t> spins, perm, onacc, ohist, ort, gsc, gss;
     std::vector<uint64_t> gsst;
     std::vector<unsigned long long> mv;
-    if ((st = d2h(oe, X.out_e, (size_t)nm * g.nch * B)) || (st = d2h(oq, X.out_q, (size_t)nm * B)) ||
-        (st = d2h(ochi, X.out_chi, (size_t)nm * B * S.nchi)) ||
-        (st = d2h(os, X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes)) ||
-        (st = d2h(spins, X.spins, (size_t)n * g.G * 2)) || (st = d2h(energy, X.energy, g.R)) ||
-        (st = d2h(perm, X.perm, (size_t)g.nch * B)) || (st = d2h(otags, X.tags, (size_t)g.nch * B)) ||
-        (st = d2h(onacc, X.nacc, (size_t)g.nch * B)) || (st = d2h(ohist, X.hist, (size_t)g.nch * B * 2)) ||
-        (st = d2h(ort, X.rt, g.nch)) || (st = d2h(gsc, X.gs_count, 1)) || (st = d2h(mv, X.icm_moves, 1))) {
+    if ((st = d2h(oe, X.out_e, (size_t)nm * g.nch * B, m)) || (st = d2h(oq, X.out_q, (size_t)nm * B, m)) ||
+        (st = d2h(ochi, X.out_chi, (size_t)nm * B * S.nchi, m)) ||
+        (st = d2h(os, X.out_samples, (size_t)S.n_samp_events * S.n_samp_slots * g.nch * S.nbytes, m)) ||
+        (st = d2h(spins, X.spins, (size_t)n * g.G * 2, m)) || (st = d2h(energy, X.energy, g.R, m)) ||
+        (st = d2h(perm, X.perm, (size_t)g.nch * B, m)) || (st = d2h(otags, X.tags, (size_t)g.nch * B, m)) ||
+        (st = d2h(onacc, X.nacc, (size_t)g.nch * B, m)) || (st = d2h(ohist, X.hist, (size_t)g.nch * B * 2, m)) ||
+        (st = d2h(ort, X.rt, g.nch, m)) || (st = d2h(gsc, X.gs_count, 1, m)) || (st = d2h(mv, X.icm_moves, 1, m))) {
       free(R);
       return st;
     }
     moves += mv[0];
     const uint32_t ngs = std::min(gsc[0], S.gs_cap);
-    if ((st = d2h(gss, X.out_gs_steps, ngs)) || (st = d2h(gse, X.out_gs_e, ngs)) ||
-        (st = d2h(gsst, X.out_gs_states, (size_t)ngs * S.gs_words))) {
+    if ((st = d2h(gss, X.out_gs_steps, ngs, m)) || (st = d2h(gse, X.out_gs_e, ngs, m)) ||
+        (st = d2h(gsst, X.out_gs_states, (size_t)ngs * S.gs_words, m))) {
       free(R);
       return st;
     }
@@ -409,8 +421,8 @@ struct RunOptions {
 // Plan, allocate, launch and assemble a set of solves that run concurrently in one kernel.
 static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &opt, tamc_pt_result **results,
                               tamc_batch_stats *stats, const std::function<void(std::vector<SolvePlan> &)> &patch = {}) {
-  Arena A;
-  tamc_status st = allocate(solves, A);
+  Arena A, Ags;  // Ags: ground-state trace buffers, large and mostly unused, downloaded by prefix
+  tamc_status st = allocate(solves, A, Ags);
   if (st) return st;
   if (patch) patch(solves);
   Launch Lc;
@@ -456,8 +468,11 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (graph) (void)hipGraphDestroy(graph);
   const double wall_us = std::chrono::duration<double, std::micro>(w1 - w0).count();
   uint64_t attempts = 0;
+  std::vector<char> mirror_buf(A.used);
+  if (A.used) TAMC_HIP(hipMemcpy(mirror_buf.data(), A.buf.p, A.used, hipMemcpyDeviceToHost));
+  const HostMirror mirror{mirror_buf.data(), (const char *)A.buf.p, A.used};
   for (size_t k = 0; k < solves.size(); ++k) {
-    if ((st = assemble(solves[k], &results[k]))) {
+    if ((st = assemble(solves[k], &results[k], &mirror))) {
       for (size_t j = 0; j < k; ++j) tamc_pt_result_free(results[j]);
       return st;
     }
@@ -467,18 +482,22 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   }
   if (stats) {
     // phase split of the group that took longest
-    unsigned long long best[6] = {0, 0, 0, 0, 0, 0}, best_tot = 0;
+    unsigned long long best[kPhaseSlots] = {0, 0, 0, 0, 0, 0, 0, 0}, best_tot = 0;
     for (auto &S : solves)
       for (auto &g : S.groups) {
-        unsigned long long tk[6];
-        if (hipMemcpy(tk, g.g.phase_ticks, sizeof tk, hipMemcpyDeviceToHost) != hipSuccess) continue;
-        const unsigned long long tot = tk[0] + tk[1] + tk[2] + tk[3] + tk[4] + tk[5];
+        unsigned long long tk[kPhaseSlots];
+        const char *src = (const char *)g.g.phase_ticks;
+        if (src < mirror.dev || src + sizeof tk > mirror.dev + mirror.bytes) continue;
+        memcpy(tk, mirror.host + (src - mirror.dev), sizeof tk);
+        unsigned long long tot = 0;
+        for (uint32_t k = 0; k < kPhIcmPasses; ++k) tot += tk[k];
         if (tot > best_tot) {
           best_tot = tot;
           memcpy(best, tk, sizeof tk);
         }
       }
-    for (int k = 0; k < 6; ++k) stats->phase_ms[k] = (double)best[k] * 1e-5;  // 100 MHz ticks
+    for (uint32_t k = 0; k < kPhIcmPasses; ++k) stats->phase_ms[k] = (double)best[k] * 1e-5;  // 100 MHz ticks
+    stats->icm_passes = (double)best[kPhIcmPasses];
     stats->device_ms = ms;
     stats->wall_ms = wall_us / 1000.0;
     stats->attempts = attempts;
@@ -558,16 +577,87 @@ tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params,
   return run_solves(solves, opt, out, nullptr);
 }
 
+// Autotuner for the batched launch: waves per work group (the task tiling of every level) is
+// chosen by timing each candidate on a short prefix of the same batch (measurements off, results
+// discarded).  Choices are cached per batch shape for the life of the process.
+namespace {
+struct TuneKey {
+  uint32_t count, max_n, B, C;
+  uint64_t sum_n;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(count, max_n, B, C, sum_n) < std::tie(o.count, o.max_n, o.B, o.C, o.sum_n);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, uint32_t> g_tune_cache;
+}  // namespace
+
+static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                                  uint32_t *waves_out) {
+  TuneKey key{count, 0, 0, 0, 0};
+  for (uint32_t k = 0; k < count; ++k) {
+    key.max_n = std::max(key.max_n, insts[k]->n);
+    key.sum_n += insts[k]->n;
+    key.B = std::max(key.B, params[k].num_betas);
+    key.C = std::max(key.C, params[k].num_replica_chains);
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      *waves_out = it->second;
+      return TAMC_OK;
+    }
+  }
+  std::vector<tamc_pt_params> probe(params, params + count);
+  for (auto &p : probe) {
+    p.num_sweeps = std::min(p.num_sweeps, 24u);
+    p.warmup_fraction = 1.0;
+    p.sample_enabled = 0;
+    p.sample_states_enabled = 0;
+  }
+  std::vector<tamc_pt_result *> tmp(count, nullptr);
+  double best_ms = 1e300;
+  uint32_t best = 16;
+  for (uint32_t w : {4u, 8u, 16u}) {
+    std::vector<SolvePlan> solves(count);
+    tamc_status st = TAMC_OK;
+    for (uint32_t k = 0; k < count && !st; ++k) st = plan_solve(solves[k], insts[k], &probe[k], nullptr, w);
+    if (st) continue;  // this tiling does not fit (LDS): skip it
+    RunOptions opt;
+    opt.waves = w;
+    tamc_batch_stats bs{};
+    if ((st = run_solves(solves, opt, tmp.data(), &bs))) return st;
+    for (auto *r : tmp) tamc_pt_result_free(r);
+    if (bs.device_ms < best_ms) {
+      best_ms = bs.device_ms;
+      best = w;
+    }
+  }
+  clear_error();
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = best;
+  *waves_out = best;
+  return TAMC_OK;
+}
+
 tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats) {
   clear_error();
   if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
+  for (uint32_t k = 0; k < count; ++k)
+    if (!insts[k]) return fail(TAMC_E_INVALID_ARG, "null instance");
   tamc_status st = device_init();
   if (st) return st;
   RunOptions opt;
   uint32_t w = 4;
-  for (uint32_t k = 0; k < count; ++k) w = std::max(w, insts[k] ? default_waves(insts[k]) : 4u);
-  opt.waves = (opts && opts->waves_per_group) ? opts->waves_per_group : w;
+  for (uint32_t k = 0; k < count; ++k) w = std::max(w, default_waves(insts[k]));
+  if (opts && opts->waves_per_group) {
+    w = opts->waves_per_group;
+  } else if (opts && opts->autotune && count) {
+    if ((st = autotune_waves(count, insts, params, &w))) return st;
+  }
+  opt.waves = w;
   if (opt.waves < 1 || opt.waves > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
   opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
   opt.use_graph = opts && opts->use_graph;

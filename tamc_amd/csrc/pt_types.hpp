@@ -15,6 +15,13 @@ constexpr uint32_t kMaxGroupReplicas = 256;
 constexpr uint32_t kMaxPairs = 32;       // ICM pairs per group held in one 32-bit cluster word
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// Per-group phase timers (100 MHz wall-clock ticks of the group's thread 0); the last slot counts
+// ICM cluster-growth passes.
+enum : uint32_t {
+  kPhIcmOverlap = 0, kPhIcmSeed, kPhIcmGrow, kPhIcmFlip, kPhExchange, kPhSweep, kPhMeasure, kPhIcmPasses,
+  kPhaseSlots
+};
+
 enum : uint32_t {
   kFlagIcm = 1u,
   kFlagThreaded = 2u,        // stale exchange dE (tamc-core/src/parallel/pt.rs:51-60)
@@ -83,7 +90,7 @@ struct PtGroup {
   long long *drift;          // [R]
   // ---- per-phase time of the group (100 MHz wall clock ticks): ICM overlap+seeds, ICM cluster growth,
   //      ICM flip, exchange, sweep, measurement ----
-  unsigned long long *phase_ticks;  // [6]
+  unsigned long long *phase_ticks;  // [kPhaseSlots]
 };
 
 struct LdsLayout {

@@ -416,7 +416,16 @@ struct tamc_server {
 
   void complete(Request *r) {
     tamc_pt_result *res = nullptr;
-    const tamc_status s = assemble(r->S, &res);
+    // one bulk download of the request's pool block, then assembly from the host copy
+    std::vector<char> host(r->pool_sz);
+    const char *dev = (const char *)pool.base + r->pool_off;
+    tamc_status s = TAMC_OK;
+    if (hipMemcpy(host.data(), dev, r->pool_sz, hipMemcpyDeviceToHost) != hipSuccess) {
+      s = TAMC_E_DEVICE;
+    } else {
+      const HostMirror m{host.data(), dev, r->pool_sz};
+      s = assemble(r->S, &res, &m);
+    }
     std::lock_guard<std::mutex> lk(mu);
     if (s) {
       fail_request(r, s, "result download failed");

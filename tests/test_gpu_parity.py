@@ -176,9 +176,11 @@ def test_batch_equals_single_and_plain(gpu):
     batch, st = api.run_pt_batch(insts, params)
     plain, _ = api.run_pt_batch(insts, params, plain=True)
     chunked, _ = api.run_pt_batch(insts, params, sweeps_per_launch=7, use_graph=True)
+    tuned, tst = api.run_pt_batch(insts, params, autotune=True)
+    assert tst["waves_per_group"] in (4, 8, 16)
     for k in range(len(insts)):
         single = api.PtIcmRunner(insts[k], params[k]).run()
-        for other in (plain[k], chunked[k], single):
+        for other in (plain[k], chunked[k], tuned[k], single):
             for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
                 assert np.array_equal(batch[k][key], other[key]), key
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
