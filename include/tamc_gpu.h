@@ -139,7 +139,7 @@ typedef struct {
 
 /* PtIcmRunner::run / run_parallel (src/pt.rs:354-412): production path.  Sweeps in an order that
  * is equivalent to the reference's sequential site order, Philox uniforms keyed by
- * (seed, sweep, site quad, replica), exchange and ICM on the device.
+ * (seed, sweep, group of four mutually uncoupled sites, replica), exchange and ICM on the device.
  * init_states (nullable) [C][B][n] replaces generate_init_state (src/pt.rs:506-519). */
 tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params, const int8_t *init_states,
                         tamc_pt_result **out);
@@ -266,6 +266,12 @@ tamc_status tamc_write_thermal_samples(const tamc_pt_result *r, const float *bet
 /* bincode-1 serialisation of AnnealState (src/sa.rs:67-89) for the states whose energy is within
  * 1e-8 of the minimum (src/sa.rs:255-270). */
 tamc_status tamc_write_anneal_state(const tamc_sa_result *r, const char *path);
+/* The same structures as Python pickles (src/pt.rs:650 and src/sa.rs:265 write serde-pickle
+ * when the sample file name ends in .pkl): protocol 3, structs as dicts keyed by field name, Vecs
+ * as lists. */
+tamc_status tamc_write_thermal_samples_pickle(const tamc_pt_result *r, const float *betas, uint8_t compression_level,
+                                              const char *path);
+tamc_status tamc_write_anneal_state_pickle(const tamc_sa_result *r, const char *path);
 
 #ifdef __cplusplus
 }

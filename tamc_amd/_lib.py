@@ -136,6 +136,8 @@ SIGNATURES = {
     "tamc_sa_result_free": (None, [C.POINTER(SaResult)]),
     "tamc_write_thermal_samples": (C.c_int, [C.POINTER(PtResult), _f32p, C.c_uint8, C.c_char_p]),
     "tamc_write_anneal_state": (C.c_int, [C.POINTER(SaResult), C.c_char_p]),
+    "tamc_write_thermal_samples_pickle": (C.c_int, [C.POINTER(PtResult), _f32p, C.c_uint8, C.c_char_p]),
+    "tamc_write_anneal_state_pickle": (C.c_int, [C.POINTER(SaResult), C.c_char_p]),
 }
 
 _lib = None

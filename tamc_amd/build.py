@@ -73,7 +73,9 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     cli_src = os.path.join(CSRC, "cli_main.cpp")
     if os.path.exists(cli_src):
-        _run([HIPCC, *COMMON, "-o", CLI, cli_src, "-L", LIB_DIR, "-ltamc_gpu", "-Wl,-rpath,$ORIGIN"])
+        # the command-line front end is plain host C++ over the C ABI
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CLI, cli_src,
+              "-L", LIB_DIR, "-ltamc_gpu", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print("built", LIB)
     return LIB

@@ -1,7 +1,11 @@
-// writers.hip -- bincode-1 serialisation of the reference's output structures (little-endian,
-// fixed-width integers, u64 length prefixes, fields in declaration order).
-//   PtIcmThermalSamples  src/pt.rs:116-125 (written at src/pt.rs:652)
+// writers.hip -- serialisation of the reference's output structures, fields in declaration order.
+//   PtIcmThermalSamples  src/pt.rs:116-125 (written at src/pt.rs:640-654)
 //   AnnealState          src/sa.rs:67-89  (written at src/sa.rs:255-270)
+// bincode-1: little-endian, fixed-width integers, u64 length prefixes.
+// Python pickle (the reference writes these with serde-pickle when the file name ends in .pkl):
+// protocol 3, a struct is a dict keyed by field name, a Vec (Vec<u8> included) is a list, integers
+// are ints and f32 values are floats.  The object a pickle loader rebuilds is what is specified;
+// serde-pickle's exact opcode choice is not reproduced.
 #include <stdio.h>
 #include <string.h>
 
@@ -25,6 +29,57 @@ struct Out {
     const uint8_t *c = (const uint8_t *)p;
     b.insert(b.end(), c, c + n);
   }
+};
+
+// Python pickle protocol 3 emitter.
+struct Pickle {
+  std::vector<uint8_t> b;
+  Pickle() {
+    b.push_back(0x80);  // PROTO
+    b.push_back(3);
+  }
+  void op(uint8_t c) { b.push_back(c); }
+  void integer(int64_t v) {
+    if (v >= 0 && v < 256) {
+      op('K');
+      op((uint8_t)v);
+    } else if (v >= 0 && v < 65536) {
+      op('M');
+      op((uint8_t)v);
+      op((uint8_t)(v >> 8));
+    } else if (v >= INT32_MIN && v <= INT32_MAX) {
+      op('J');
+      const int32_t w = (int32_t)v;
+      b.insert(b.end(), (const uint8_t *)&w, (const uint8_t *)&w + 4);
+    } else {
+      op(0x8a);  // LONG1: little-endian two's complement
+      op(8);
+      b.insert(b.end(), (const uint8_t *)&v, (const uint8_t *)&v + 8);
+    }
+  }
+  void real(double v) {
+    op('G');  // BINFLOAT, big-endian
+    uint64_t u;
+    memcpy(&u, &v, 8);
+    for (int k = 7; k >= 0; --k) op((uint8_t)(u >> (8 * k)));
+  }
+  void str(const char *s) {
+    const uint32_t n = (uint32_t)strlen(s);
+    op('X');
+    b.insert(b.end(), (const uint8_t *)&n, (const uint8_t *)&n + 4);
+    b.insert(b.end(), (const uint8_t *)s, (const uint8_t *)s + n);
+  }
+  void dict_begin() {
+    op('}');
+    op('(');
+  }
+  void dict_end() { op('u'); }
+  void list_begin() {
+    op(']');
+    op('(');
+  }
+  void list_end() { op('e'); }
+  void stop() { op('.'); }
 };
 
 tamc_status write_file(const char *path, const std::vector<uint8_t> &b) {
@@ -100,6 +155,98 @@ tamc_status tamc_write_anneal_state(const tamc_sa_result *r, const char *path) {
   o.u32(cnt);
   o.u64(bytes.size());
   o.raw(bytes.data(), bytes.size());
+  return write_file(path, o.b);
+}
+
+tamc_status tamc_write_thermal_samples_pickle(const tamc_pt_result *r, const float *betas, uint8_t compression_level,
+                                              const char *path) {
+  clear_error();
+  if (!r || !betas || !path) return fail(TAMC_E_INVALID_ARG, "null argument");
+  const uint32_t B = r->B, C = r->C, nm = r->n_meas, np = r->npairs;
+  Pickle o;
+  o.dict_begin();
+  o.str("compression_level");
+  o.integer(compression_level);
+  o.str("instance_size");
+  o.integer(r->n);
+  o.str("beta_arr");
+  o.list_begin();
+  for (uint32_t b = 0; b < B; ++b) o.real(betas[b]);
+  o.list_end();
+  o.str("samples");
+  o.list_begin();
+  const uint64_t per = (uint64_t)r->n_samp_events * C;
+  for (uint32_t s = 0; s < r->n_samp_slots; ++s) {
+    o.list_begin();
+    for (uint64_t k = 0; k < per; ++k) {
+      o.list_begin();
+      const uint8_t *p = r->samples + ((uint64_t)s * per + k) * r->nbytes;
+      for (uint32_t x = 0; x < r->nbytes; ++x) o.integer(p[x]);
+      o.list_end();
+    }
+    o.list_end();
+  }
+  o.list_end();
+  o.str("e");
+  o.list_begin();
+  for (uint32_t b = 0; b < B; ++b) {
+    o.list_begin();
+    for (uint64_t k = 0; k < (uint64_t)nm * C; ++k) o.real(r->e[(uint64_t)b * nm * C + k]);
+    o.list_end();
+  }
+  o.list_end();
+  o.str("q");
+  o.list_begin();
+  for (uint32_t b = 0; b < B; ++b) {
+    o.list_begin();
+    for (uint64_t k = 0; k < (uint64_t)nm * np; ++k) o.integer(r->q[(uint64_t)b * nm * np + k]);
+    o.list_end();
+  }
+  o.list_end();
+  o.str("suscept");
+  o.list_begin();
+  for (uint32_t b = 0; b < B; ++b) {
+    o.list_begin();
+    for (uint32_t c = 0; c < r->nchi; ++c) {
+      o.list_begin();
+      for (uint64_t k = 0; k < (uint64_t)nm * np; ++k) o.real(r->suscept[((uint64_t)b * r->nchi + c) * nm * np + k]);
+      o.list_end();
+    }
+    o.list_end();
+  }
+  o.list_end();
+  o.dict_end();
+  o.stop();
+  return write_file(path, o.b);
+}
+
+tamc_status tamc_write_anneal_state_pickle(const tamc_sa_result *r, const char *path) {
+  clear_error();
+  if (!r || !path) return fail(TAMC_E_INVALID_ARG, "null argument");
+  const uint32_t n = r->n, nb = (n + 7) / 8;
+  const float thr = r->min_energy + 1.0e-8f;
+  std::vector<uint8_t> bytes;
+  uint32_t cnt = 0;
+  for (uint32_t k = 0; k < r->R; ++k) {
+    if (!(r->energies[k] <= thr)) continue;
+    ++cnt;
+    const size_t o = bytes.size();
+    bytes.resize(o + nb, 0);
+    for (uint32_t i = 0; i < n; ++i)
+      if (!(r->final_states[(size_t)k * n + i] > 0)) bytes[o + i / 8] |= (uint8_t)(1u << (i % 8));
+  }
+  Pickle o;
+  o.dict_begin();
+  o.str("instance_size");
+  o.integer(n);
+  o.str("num_replicas");
+  o.integer(cnt);
+  o.str("bytes");
+  o.list_begin();
+  for (uint8_t v : bytes) o.integer(v);
+  o.list_end();
+  o.dict_end();
+  o.stop();
   return write_file(path, o.b);
 }
 

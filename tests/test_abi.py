@@ -138,3 +138,22 @@ def test_thermal_samples_bincode_layout():
         assert take("Q") == 1
         assert list(np.atleast_1d(take("%df" % take("Q")))) == list(chi[b * 2:(b + 1) * 2])
     assert off == len(data)
+
+
+def test_thermal_samples_pickle_matches_bincode_fields():
+    import pickle
+    R, keep, (e, q, chi, smp) = _fake_result()
+    betas = np.array([0.5, 1.0, 2.0], np.float32)
+    path = tempfile.mktemp(suffix=".pkl")
+    _lib.check(_lib.lib().tamc_write_thermal_samples_pickle(C.byref(R), betas.ctypes.data_as(_lib._f32p), 2,
+                                                            path.encode()))
+    with open(path, "rb") as f:
+        obj = pickle.load(f)  # written by this library
+    os.unlink(path)
+    assert list(obj) == ["compression_level", "instance_size", "beta_arr", "samples", "e", "q", "suscept"]
+    assert obj["compression_level"] == 2 and obj["instance_size"] == 10
+    assert obj["beta_arr"] == [float(b) for b in betas]
+    assert len(obj["samples"]) == 1 and len(obj["samples"][0]) == 4 and obj["samples"][0][1] == list(smp[2:4])
+    assert obj["e"] == [[float(x) for x in e[k * 4:(k + 1) * 4]] for k in range(3)]
+    assert obj["q"] == [[int(x) for x in q[k * 2:(k + 1) * 2]] for k in range(3)]
+    assert obj["suscept"] == [[[float(x) for x in chi[b * 2:(b + 1) * 2]]] for b in range(3)]
