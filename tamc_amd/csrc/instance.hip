@@ -206,11 +206,11 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
     return nullptr;
   }
   if (!D->ell_col.p) {
-    // fixed-stride rows (ELL) when every row has at most 8 entries: a quad's couplings then load
-    // with a few wide scalar loads; padding adds +0.0, which leaves every fp32 sum unchanged
+    // fixed-stride rows (ELL) of width 3, 4, 6 or 8 when every row fits: a group's couplings then
+    // load with a few wide scalar loads; padding adds +0.0, which leaves every fp32 sum unchanged
     uint32_t dmax = 0;
     for (uint32_t i = 0; i < inst->n; ++i) dmax = std::max(dmax, inst->row_ptr[i + 1] - inst->row_ptr[i]);
-    const uint32_t Dl = dmax <= 4 ? 4u : (dmax <= 8 ? 8u : 0u);
+    const uint32_t Dl = dmax <= 3 ? 3u : dmax <= 4 ? 4u : dmax <= 6 ? 6u : dmax <= 8 ? 8u : 0u;
     std::vector<uint32_t> ec((size_t)inst->n * std::max(Dl, 1u));
     std::vector<float> eJ(ec.size(), 0.0f);
     for (uint32_t i = 0; i < inst->n && Dl; ++i)

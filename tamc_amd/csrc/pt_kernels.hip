@@ -51,15 +51,20 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
 
 __device__ __forceinline__ ExpfExceptions exc_table() { return ExpfExceptions{kExpfExcX, kExpfExcY, kExpfExcCount}; }
 
-// u < expf(x) for u = (w >> 8) * 2^-24, decided with the hardware exp2 unless u falls within a
-// relative 2^-12 band of the estimate; inside the band the exact double-precision path decides.
+// Relative error band of the hardware estimate p = v_exp_f32(x * log2(e)) of expf(x): rounding
+// x * log2(e) costs |x| 2^-23 and v_exp_f32 2^-23, together below 2^-16.3 for |x| <= 105 (expf
+// underflows to 0 below -104).  The band 2^-14 leaves a factor 4.9 of margin.
+constexpr float kBand = 0x1p-14f;
+
+// u < expf(x) for u = (w >> 8) * 2^-24, decided with the hardware exp2 unless u falls within the
+// band around the estimate; inside the band the exact double-precision path decides.
 __device__ __forceinline__ bool accept_u(float x, uint32_t w) {
   const uint32_t k = w >> 8;
   const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   const float P = p * 16777216.0f;
   const float kf = (float)k;
-  if (kf < P * (1.0f - 0x1p-12f)) return true;
-  if (kf > P * (1.0f + 0x1p-12f)) return false;
+  if (kf < P * (1.0f - kBand)) return true;
+  if (kf > P * (1.0f + kBand)) return false;
   return accept_exact(x, k, exc_table());
 }
 
@@ -411,13 +416,21 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
   }
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
   if (P <= 16u) {
-    if (X.ell_D == 4u) icm_flip<4, 16>(c, M);
-    else if (X.ell_D == 8u) icm_flip<8, 16>(c, M);
-    else icm_flip<0, 16>(c, M);
+    switch (X.ell_D) {
+      case 3u: icm_flip<3, 16>(c, M); break;
+      case 4u: icm_flip<4, 16>(c, M); break;
+      case 6u: icm_flip<6, 16>(c, M); break;
+      case 8u: icm_flip<8, 16>(c, M); break;
+      default: icm_flip<0, 16>(c, M);
+    }
   } else {
-    if (X.ell_D == 4u) icm_flip<4, 32>(c, M);
-    else if (X.ell_D == 8u) icm_flip<8, 32>(c, M);
-    else icm_flip<0, 32>(c, M);
+    switch (X.ell_D) {
+      case 3u: icm_flip<3, 32>(c, M); break;
+      case 4u: icm_flip<4, 32>(c, M); break;
+      case 6u: icm_flip<6, 32>(c, M); break;
+      case 8u: icm_flip<8, 32>(c, M); break;
+      default: icm_flip<0, 32>(c, M);
+    }
   }
   __syncthreads();
   // moved pairs: energies become energy_ref of the new states, as the reference recomputes them
@@ -524,8 +537,12 @@ __device__ __forceinline__ float site_field(const Ctx &c, uint32_t i, uint32_t g
       Jv[k] = jp[k];
       w[k] = c.sp[(cp[k] * G + g) * 2u + half];
     }
+    // d += s_j J_j as an fma with s_j = +-1.0 built from the lane's spin bit: J_j s_j is exact, so
+    // the fma rounds exactly like the reference's separate add (zeros' signs included)
+    const uint32_t up = 31u - sh;
 #pragma unroll
-    for (int k = 0; k < D; ++k) d += ((w[k] >> sh) & 1u) ? -Jv[k] : Jv[k];
+    for (int k = 0; k < D; ++k)
+      d = __fmaf_rn(Jv[k], __uint_as_float(0x3f800000u | ((w[k] << up) & 0x80000000u)), d);
   } else {
     const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
     for (uint32_t e = e0; e < e1; ++e) {
@@ -538,18 +555,24 @@ __device__ __forceinline__ float site_field(const Ctx &c, uint32_t i, uint32_t g
   return d;
 }
 
-// 3. Metropolis sweep, level by level
+// 3. Metropolis sweep, level by level.  Tasks are numbered level-major, word-minor: task k is word
+// k % G of RNG group k / G.  Each task decides its four sites in two passes: a straight-line pass
+// with the hardware exp2 (accept when u is below the estimate by more than the error band, reject
+// when above), and, only when some lane of the wave fell inside the band, the exact test.
+
 template <int D>
 __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
-  const uint32_t G = c.G, n = c.n;
+  const uint32_t G = c.G;
   long long acc[4] = {0, 0, 0, 0};
   const uint32_t half = c.lane >> 5, sh = c.lane & 31u;
-  for (uint32_t l = 0; l < X.n_levels; ++l) {
-    const uint32_t a = c4(X.lvl_off)[l], b = c4(X.lvl_off)[l + 1];
+  const uint32_t nl = X.n_levels;
+  const bool self = X.self_J != nullptr;
+  uint32_t a = c4(X.lvl_off)[0];
+  for (uint32_t l = 0; l < nl; ++l) {
+    const uint32_t b = c4(X.lvl_off)[l + 1];
     for (uint32_t k = a + c.wave; k < b; k += c.W) {
-      const uint32_t task = uni(c4(X.tasks)[k]);
-      const uint32_t grp = task / G, g = task - grp * G;
+      const uint32_t grp = (G == 1u) ? k : k / G, g = k - grp * G;
       const uint32_t r = g * 64u + c.lane;
       const bool valid = r < c.R;
       const float beta = valid ? c.beta_of[r] : 0.0f;
@@ -571,19 +594,37 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
           bi[s] = (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u;
         }
       }
+      float dd[4], xs[4];
+      uint32_t ks[4];
+      bool ok[4], band[4];
+      bool any_band = false;
+#pragma unroll
+      for (uint32_t s = 0; s < 4u; ++s) {
+        dd[s] = d[s] * (bi[s] ? 2.0f : -2.0f);  // dE *= -2*s_i
+        xs[s] = -beta * dd[s];
+        ks[s] = pick(u, s) >> 8;
+        const float P = __builtin_amdgcn_exp2f(xs[s] * 1.44269504088896341f) * 16777216.0f;
+        const float kf = (float)ks[s];
+        const bool neg = dd[s] < 0.0f, lo = kf < P * (1.0f - kBand), hi = kf > P * (1.0f + kBand);
+        ok[s] = neg || lo;
+        band[s] = !neg && !lo && !hi && valid && site[s] != kNone;
+        any_band |= band[s];
+      }
+      if (ballot(any_band) != 0ull) {
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s)
+          if (band[s]) ok[s] = accept_exact(xs[s], ks[s], exc_table());
+      }
       float a_g = 0.f;
 #pragma unroll
       for (uint32_t s = 0; s < 4u; ++s) {
         if (site[s] == kNone) break;
         const uint32_t i = site[s];
-        float dd = d[s] * (bi[s] ? 2.0f : -2.0f);  // dE *= -2*s_i
-        bool acc_ok = dd < 0.0f;
-        if (!acc_ok) acc_ok = accept_u(-beta * dd, pick(u, s));
-        acc_ok = acc_ok && valid;
+        const bool acc_ok = ok[s] && valid;
         const uint64_t nw = ballot((bi[s] ^ (acc_ok ? 1u : 0u)) != 0u);
         if (sh == 0u) c.sp[(i * G + g) * 2u + half] = half ? (uint32_t)(nw >> 32) : (uint32_t)nw;
-        if (acc_ok) a_g += dd;
-        if (X.self_J != nullptr) {
+        if (acc_ok) a_g += dd[s];
+        if (self) {
           const float sj = c4(X.self_J)[i];
           if (acc_ok && sj != 0.0f) atomicAdd((unsigned long long *)&c.drift[r], (unsigned long long)to_fix(-2.0f * sj));
         }
@@ -594,6 +635,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       for (uint32_t gg = 0; gg < 4u; ++gg)
         if (gg == g) acc[gg] += fx;
     }
+    a = b;
     __syncthreads();
   }
 #pragma unroll
@@ -610,10 +652,13 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
 }
 
 __device__ __forceinline__ void sweep_phase(Ctx &c, uint32_t t) {
-  const uint32_t D = c.X->ell_D;
-  if (D == 4u) sweep_phase_d<4>(c, t);
-  else if (D == 8u) sweep_phase_d<8>(c, t);
-  else sweep_phase_d<0>(c, t);
+  switch (c.X->ell_D) {
+    case 3u: sweep_phase_d<3>(c, t); break;
+    case 4u: sweep_phase_d<4>(c, t); break;
+    case 6u: sweep_phase_d<6>(c, t); break;
+    case 8u: sweep_phase_d<8>(c, t); break;
+    default: sweep_phase_d<0>(c, t);
+  }
 }
 
 __device__ __forceinline__ uint32_t event_index(uint32_t stp, uint32_t S) {
