@@ -111,6 +111,72 @@ def serving_leg(n_req, rate, sweeps, slice_ms):
     }
 
 
+def other_configs(with_cpu):
+    """The reference's other configurations, each next to the CPU restatement (one thread, the
+    reference's serial loop with Xoshiro256++) on a bounded sample:
+      plain_2d: 2D L=16 periodic +-J, 32 betas 0.2-5.0, 2 chains, 2000 sweeps (one instance, plain path)
+      plain_3d: 3D L=12 periodic N(0,1), 48 betas 0.1-5.0, 4 chains, 10000 sweeps (one instance)
+      sa_qubo:  64 QUBOs N=2048, 5% density, Q ~ U{-100..100}, 1024 restarts each, linear beta
+                0.1 -> 10 over 4096 sweeps; timed on the first 64 sweeps of the schedule."""
+    from tamc_amd import api, workloads as W
+    out = {}
+
+    def pt_case(name, lines, B, bmin, bmax, C, T, cpu_sweeps):
+        i, j, K, n = lines
+        inst = api.BqmIsingInstance.from_lines(i, j, K)
+        p = api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(bmin, bmax, B),
+                            lo_beta=1.0, icm=True, num_replica_chains=C, sample=32, sample_states=32,
+                            sample_limiting=2, seed=1)
+        api.PtIcmRunner(inst, p).run()  # warm (instance upload)
+        t0 = time.perf_counter()
+        r = api.PtIcmRunner(inst, p).run()
+        dt = time.perf_counter() - t0
+        rec = {"value": r["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(r["attempts"]),
+               "ms": dt * 1e3, "kernel_ms": r["device_ms"], "n": n}
+        if with_cpu:
+            import oracle as O
+            betas, lo = p.resolve()
+            oi = O.Instance.from_lines(i, j, K)
+            t0 = time.perf_counter()
+            o = O.pt_run(oi, betas, cpu_sweeps, num_chains=C, warmup_fraction=0.5, lo_beta_idx=lo, icm=True,
+                         sample=32, sample_states=32, sample_limiting=2, rng_mode=O.RNG_XOSHIRO, seed=1)
+            cdt = time.perf_counter() - t0
+            rec["cpu_1thread"] = {"value": o["attempts"] / cdt, "cores": 1,
+                                  "sample": "%d of %d sweeps" % (cpu_sweeps, T)}
+            rec["vs_cpu_1thread"] = rec["value"] / rec["cpu_1thread"]["value"]
+        out[name] = rec
+
+    pt_case("plain_2d", W.lattice_2d(16, np.random.default_rng(101), pm_j=True), 32, 0.2, 5.0, 2, 2000, 2000)
+    pt_case("plain_3d", W.lattice_3d(12, np.random.default_rng(102), pm_j=False), 48, 0.1, 5.0, 4, 10000, 100)
+
+    insts, lines = [], []
+    for k in range(64):
+        q = W.qubo_sparse(2048, 0.05, np.random.default_rng(104 + k), qmax=100)
+        lines.append(q)
+        insts.append(api.BqmIsingInstance.from_lines(q[0], q[1], q[2], qubo=True))
+    S = 64
+    betas = (0.1 + 9.9 * np.arange(4096) / 4095).astype(np.float32)
+    ps = [api.SaParams(beta=api.BetaOptions.Arr(betas[:S]), num_replicas=1024, threads=1, seed=k) for k in range(64)]
+    api.run_sa_batch(insts[:1], ps[:1])  # warm
+    t0 = time.perf_counter()
+    _, st = api.run_sa_batch(insts, ps)
+    dt = time.perf_counter() - t0
+    rec = {"value": st["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(st["attempts"]),
+           "ms": dt * 1e3, "kernel_ms": st["device_ms"],
+           "sample": "all 64 instances x 1024 restarts, first %d of 4096 sweeps" % S,
+           "roofline_algorithmic_GBs": st["attempts"] / dt * 939 / 1e9}
+    if with_cpu:
+        import oracle as O
+        oi = O.Instance.from_lines(lines[0][0], lines[0][1], lines[0][2], qubo=True)
+        t0 = time.perf_counter()
+        o = O.sa_run(oi, betas[:S], 64, threads=1, rng_mode=O.RNG_XOSHIRO, seed=1)
+        cdt = time.perf_counter() - t0
+        rec["cpu_1thread"] = {"value": o["attempts"] / cdt, "cores": 1, "sample": "1 instance x 64 restarts x %d sweeps" % S}
+        rec["vs_cpu_1thread"] = rec["value"] / rec["cpu_1thread"]["value"]
+    out["sa_qubo"] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +193,7 @@ def main():
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the other configurations")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-sweeps", type=int, default=2000)
@@ -225,6 +292,8 @@ def main():
                      "algorithmic_bytes_per_launch": attempts_per_step * BYTES_PER_ATTEMPT_D3,
                      "kernel_ms": float(np.mean(dev_ms))},
     }
+    if not args.no_other:
+        out["other_configs"] = other_configs(not args.no_cpu)
     if not args.no_serving:
         out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_sweeps, args.serve_slice_ms)
     if not args.no_cpu:

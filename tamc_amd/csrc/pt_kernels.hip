@@ -392,8 +392,8 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     int changed = 0;
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
       const uint32_t q = M.qw[i];
-      if (!q) continue;
       const uint32_t cur = M.cw[i];
+      if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
       uint32_t nb = 0;
       if (D) {
         for (uint32_t k = 0; k < D; ++k) nb |= M.cw[M.nbr((size_t)i * D + k)];
