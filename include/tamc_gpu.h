@@ -101,7 +101,15 @@ typedef struct {
   uint32_t sample_states;
   uint32_t sample_limiting; /* Option<u8>, 0 when None */
   uint64_t seed;            /* production Philox key; results depend only on (instance, params, seed) */
+  uint32_t sweep_order;     /* TAMC_ORDER_COLOUR (production default) or TAMC_ORDER_SEQUENTIAL */
 } tamc_pt_params;
+
+/* Site order of a production sweep.  COLOUR: colour classes of a greedy colouring in site order
+ * (colour(i) = the smallest colour not taken by a lower-numbered coupled site), swept class by
+ * class; a valid Metropolis sweep with the same stationary distribution as the reference's.
+ * SEQUENTIAL: classes = levels (level(i) = 1 + max level of lower-numbered coupled sites), which
+ * reproduces the reference's sequential site order exactly (tamc-core/src/metropolis.rs:77-83). */
+enum { TAMC_ORDER_COLOUR = 0, TAMC_ORDER_SEQUENTIAL = 1 };
 
 /* PtIcmRunner::new (src/pt.rs:307-351): geometric betas (tamc-core/src/sa.rs:13-22, cast to f32) or an
  * explicit array, the non-decreasing check and lo_beta_idx.  lo_num_beta < 0 and lo_beta NaN mean None.

@@ -30,6 +30,7 @@ class PtParams(C.Structure):
         ("sample_states_enabled", C.c_int32), ("sample_states", C.c_uint32), ("sample_limiting", C.c_uint32),
         ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("num_streams", C.c_uint32),
         ("stream_words", C.POINTER(_u32p)), ("stream_lens", _u64p), ("init_states", _i8p),
+        ("sweep_order", C.c_uint32),
     ]
 
 
@@ -290,7 +291,7 @@ def _arr(ptr, count, dtype):
 
 def pt_run(inst, betas, num_sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_idx=0, icm=True, threaded=False,
            sample=None, sample_states=None, sample_limiting=0, rng_mode=RNG_PHILOX, seed=0, streams=None,
-           init_states=None):
+           init_states=None, sweep_order="colour"):
     """Run PT-ICM with the restated loop; returns a dict of numpy arrays (slot order)."""
     betas = np.ascontiguousarray(betas, np.float32)
     B = len(betas)
@@ -310,6 +311,7 @@ def pt_run(inst, betas, num_sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_i
     P.sample_limiting = sample_limiting
     P.rng_mode = rng_mode
     P.seed = seed
+    P.sweep_order = {"colour": 0, "sequential": 1}[sweep_order]
     keep = []
     if streams is not None:
         arrs = [np.ascontiguousarray(s, np.uint32) for s in streams]

@@ -289,9 +289,10 @@ inline float delta_energy(const Inst &I, const int8_t *s, uint32_t i) {
 
 // tamc-core/src/metropolis.rs:43-57 inside src/ising.rs:225-233; U(i) is called only when dE >= 0
 template <class U>
-float sweep(const Inst &I, int8_t *s, float beta, U &&draw) {
+float sweep(const Inst &I, int8_t *s, float beta, U &&draw, const uint32_t *order = nullptr) {
   float de = 0.0f;
-  for (uint32_t i = 0; i < I.n; ++i) {
+  for (uint32_t k = 0; k < I.n; ++k) {
+    const uint32_t i = order ? order[k] : k;
     const float d = delta_energy(I, s, i);
     if (d < 0.0f) {
       s[i] = (int8_t)-s[i];
@@ -308,29 +309,47 @@ float sweep(const Inst &I, int8_t *s, float beta, U &&draw) {
   return de;
 }
 
-// Production RNG grouping of the GPU path (not a reference concept): level(i) = 1 + max level of the
-// lower-numbered sites in row i (0 if none); within a level, sites in ascending order form groups of
-// four numbered level-major.  The PT uniform for (sweep t, site i, replica rho) is word slot(i) of
-// Philox4x32-10(ctr = {t, group(i), rho, 0}).
+// Production sweep schedule of the GPU path (not a reference concept).  Sites are split into
+// classes of mutually uncoupled sites and swept class by class, ascending inside a class:
+//   order 0 (colour): greedy colouring in site order, colour(i) = the smallest colour not taken by
+//            a lower-numbered site in row i;
+//   order 1 (sequential): level(i) = 1 + max level of the lower-numbered sites in row i (0 if
+//            none); sweeping level by level is the same as sweeping 0..n-1.
+// Within a class, sites in ascending order form groups of four numbered class-major.  The PT
+// uniform for (sweep t, site i, replica rho) is word slot(i) of Philox4x32-10(ctr = {t, group(i),
+// rho, 0}).
 struct RngGroups {
-  std::vector<uint32_t> gid, slot;
-  explicit RngGroups(const Inst &I) : gid(I.n), slot(I.n) {
-    std::vector<uint32_t> lv(I.n, 0);
-    uint32_t maxl = 0;
+  std::vector<uint32_t> gid, slot, order;
+  RngGroups(const Inst &I, uint32_t sweep_order) : gid(I.n), slot(I.n) {
+    std::vector<uint32_t> lv(I.n, 0), col(I.n, 0);
+    uint32_t maxl = 0, maxc = 0;
     for (uint32_t i = 0; i < I.n; ++i) {
       uint32_t l = 0;
-      for (uint32_t e = I.row_ptr[i]; e < I.row_ptr[i + 1]; ++e)
-        if (I.col[e] < i) l = std::max(l, lv[I.col[e]] + 1);
+      std::vector<bool> taken;
+      for (uint32_t e = I.row_ptr[i]; e < I.row_ptr[i + 1]; ++e) {
+        const uint32_t j = I.col[e];
+        if (j < i) {
+          l = std::max(l, lv[j] + 1);
+          if (col[j] >= taken.size()) taken.resize(col[j] + 1, false);
+          taken[col[j]] = true;
+        }
+      }
+      uint32_t c = 0;
+      while (c < taken.size() && taken[c]) ++c;
       lv[i] = l;
+      col[i] = c;
       maxl = std::max(maxl, l);
+      maxc = std::max(maxc, c);
     }
-    std::vector<std::vector<uint32_t>> by(I.n ? maxl + 1 : 0);
-    for (uint32_t i = 0; i < I.n; ++i) by[lv[i]].push_back(i);
+    const std::vector<uint32_t> &cls = sweep_order == 0 ? col : lv;
+    std::vector<std::vector<uint32_t>> by(I.n ? (sweep_order == 0 ? maxc : maxl) + 1 : 0);
+    for (uint32_t i = 0; i < I.n; ++i) by[cls[i]].push_back(i);
     uint32_t base = 0;
     for (auto &v : by) {
       for (size_t k = 0; k < v.size(); ++k) {
         gid[v[k]] = base + (uint32_t)(k / 4);
         slot[v[k]] = (uint32_t)(k % 4);
+        order.push_back(v[k]);
       }
       base += (uint32_t)((v.size() + 3) / 4);
     }
@@ -507,7 +526,7 @@ struct PtRun {
   uint64_t icm_moves = 0, attempts = 0;
   RngGroups groups;
 
-  PtRun(const Inst &I_, const tamco_pt_params &P_) : I(I_), P(P_), groups(I_) {
+  PtRun(const Inst &I_, const tamco_pt_params &P_) : I(I_), P(P_), groups(I_, P_.sweep_order) {
     n = I.n;
     B = P.num_betas;
     C = P.num_chains;
@@ -619,7 +638,8 @@ struct PtRun {
       St &x = ch.st[b];
       float de;
       if (P.rng_mode == TAMCO_RNG_PHILOX) {
-        de = sweep(I, x.s.data(), beta[b], PhiloxGroupDraw{P.seed, t, x.id, &groups});
+        de = sweep(I, x.s.data(), beta[b], PhiloxGroupDraw{P.seed, t, x.id, &groups},
+                   P.sweep_order == 0 ? groups.order.data() : nullptr);
       } else {
         WordStream &ws = streams[stream_index(c, b)];
         de = sweep(I, x.s.data(), beta[b], [&](uint32_t) { return ws.next_f32(); });

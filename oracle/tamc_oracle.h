@@ -111,6 +111,9 @@ typedef struct {
   const uint32_t *const *stream_words;
   const uint64_t *stream_lens;
   const int8_t *init_states; /* nullable, [C][B][n] slot order */
+  /* TAMCO_RNG_PHILOX only: 0 = sweep colour class by colour class (the GPU's production order),
+   * 1 = the reference's sequential order (0..n-1) */
+  uint32_t sweep_order;
 } tamco_pt_params;
 
 typedef struct {

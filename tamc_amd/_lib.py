@@ -40,7 +40,7 @@ class PtParams(C.Structure):
                 ("betas", _f32p), ("lo_beta_idx", C.c_uint32), ("icm", C.c_int32),
                 ("num_replica_chains", C.c_uint32), ("threads", C.c_uint32), ("sample_enabled", C.c_int32),
                 ("sample", C.c_uint32), ("sample_states_enabled", C.c_int32), ("sample_states", C.c_uint32),
-                ("sample_limiting", C.c_uint32), ("seed", C.c_uint64)]
+                ("sample_limiting", C.c_uint32), ("seed", C.c_uint64), ("sweep_order", C.c_uint32)]
 
 
 class PtResult(C.Structure):

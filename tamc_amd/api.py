@@ -145,6 +145,7 @@ class PtIcmParams:
     sample_states: Optional[int] = 64
     sample_limiting: Optional[int] = 0
     seed: int = 0
+    sweep_order: str = "colour"  # "colour" (production) or "sequential" (the reference's site order)
 
     def resolve(self):
         """PtIcmRunner::new (src/pt.rs:307-351): (betas f32 array, lo_beta_idx)."""
@@ -177,6 +178,7 @@ class PtIcmParams:
         P.sample_states = self.sample_states or 0
         P.sample_limiting = self.sample_limiting or 0
         P.seed = self.seed
+        P.sweep_order = {"colour": 0, "sequential": 1}[self.sweep_order]
         return P
 
 

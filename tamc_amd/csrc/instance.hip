@@ -107,30 +107,44 @@ static void adj_put(Adj &adj, uint32_t i, uint32_t j, double K, bool verbose) {
   }
 }
 
-static void compute_levels(tamc_instance *I) {
+static void group_classes(SiteSchedule &S, const std::vector<uint32_t> &cls, uint32_t ncls) {
+  std::vector<std::vector<uint32_t>> by(ncls);
+  for (uint32_t i = 0; i < (uint32_t)cls.size(); ++i) by[cls[i]].push_back(i);
+  S.n_classes = ncls;
+  S.class_groups.assign(ncls + 1, 0u);
+  S.group_sites.clear();
+  for (uint32_t l = 0; l < ncls; ++l) {
+    for (size_t k = 0; k < by[l].size(); k += 4)
+      for (size_t s = 0; s < 4; ++s) S.group_sites.push_back(k + s < by[l].size() ? by[l][k + s] : kNone);
+    S.class_groups[l + 1] = (uint32_t)(S.group_sites.size() / 4);
+  }
+  S.n_groups = (uint32_t)(S.group_sites.size() / 4);
+}
+
+static void compute_schedules(tamc_instance *I) {
   const uint32_t n = I->n;
-  std::vector<uint32_t> lv(n, 0u);
-  uint32_t maxl = 0;
+  std::vector<uint32_t> lv(n, 0u), col(n, 0u);
+  uint32_t maxl = 0, maxc = 0;
+  std::vector<uint8_t> used;
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t l = 0;
+    used.assign(I->row_ptr[i + 1] - I->row_ptr[i] + 2, 0u);
     for (uint32_t e = I->row_ptr[i]; e < I->row_ptr[i + 1]; ++e) {
       const uint32_t j = I->col[e];
-      if (j < i) l = std::max(l, lv[j] + 1u);
+      if (j < i) {
+        l = std::max(l, lv[j] + 1u);
+        if (col[j] < used.size()) used[col[j]] = 1;
+      }
     }
+    uint32_t c = 0;
+    while (used[c]) ++c;
     lv[i] = l;
+    col[i] = c;
     maxl = std::max(maxl, l);
+    maxc = std::max(maxc, c);
   }
-  I->n_levels = n ? maxl + 1u : 0u;
-  std::vector<std::vector<uint32_t>> by(I->n_levels);
-  for (uint32_t i = 0; i < n; ++i) by[lv[i]].push_back(i);
-  I->level_groups.assign(I->n_levels + 1, 0u);
-  I->group_sites.clear();
-  for (uint32_t l = 0; l < I->n_levels; ++l) {
-    for (size_t k = 0; k < by[l].size(); k += 4)
-      for (size_t s = 0; s < 4; ++s) I->group_sites.push_back(k + s < by[l].size() ? by[l][k + s] : kNone);
-    I->level_groups[l + 1] = (uint32_t)(I->group_sites.size() / 4);
-  }
-  I->n_groups = (uint32_t)(I->group_sites.size() / 4);
+  group_classes(I->sched[kOrderSequential], lv, n ? maxl + 1u : 0u);
+  group_classes(I->sched[kOrderColour], col, n ? maxc + 1u : 0u);
 }
 
 static tamc_instance *build(const Adj &adj, int qubo) {
@@ -179,11 +193,11 @@ static tamc_instance *build(const Adj &adj, int qubo) {
   I->g_col.resize(pat.size());
   std::vector<uint32_t> fill(I->g_row_ptr.begin(), I->g_row_ptr.end());
   for (auto &e : pat) I->g_col[fill[e.first]++] = e.second;
-  compute_levels(I);
+  compute_schedules(I);
   return I;
 }
 
-DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_status *st) {
+DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_status *st, uint32_t order) {
   tamc_instance *inst = const_cast<tamc_instance *>(cinst);
   std::lock_guard<std::mutex> lk(inst->mu);
   *st = TAMC_OK;
@@ -240,32 +254,34 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
       return nullptr;
     }
   }
-  if (!D->sched.count(G)) {
+  const uint32_t key = 2u * G + order;
+  if (!D->sched.count(key)) {
+    const SiteSchedule &H = inst->sched[order];
     std::unique_ptr<DeviceSchedule> S(new DeviceSchedule());
-    // tasks (group, word): level-major, word-minor
-    std::vector<uint32_t> cnt(inst->n_levels + 1, 0u), tasks;
-    for (uint32_t l = 0; l < inst->n_levels; ++l) {
-      for (uint32_t q = inst->level_groups[l]; q < inst->level_groups[l + 1]; ++q)
+    // tasks (group, word): class-major, word-minor, so task k is word k % G of group k / G
+    std::vector<uint32_t> cnt(H.n_classes + 1, 0u), tasks;
+    for (uint32_t l = 0; l < H.n_classes; ++l) {
+      for (uint32_t q = H.class_groups[l]; q < H.class_groups[l + 1]; ++q)
         for (uint32_t g = 0; g < G; ++g) tasks.push_back(q * G + g);
       cnt[l + 1] = (uint32_t)tasks.size();
     }
-    S->n_levels = inst->n_levels;
+    S->n_levels = H.n_classes;
     S->n_tasks = (uint32_t)tasks.size();
     if (!up(S->lvl_off, cnt.data(), cnt.size() * 4) || !up(S->tasks, tasks.data(), tasks.size() * 4) ||
-        !up(S->groups, inst->group_sites.data(), inst->group_sites.size() * 4)) {
+        !up(S->groups, H.group_sites.data(), H.group_sites.size() * 4)) {
       *st = fail(TAMC_E_DEVICE, "schedule upload failed");
       return nullptr;
     }
-    D->sched[G] = std::move(S);
+    D->sched[key] = std::move(S);
   }
   return D;
 }
 
-const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st) {
-  DeviceInstance *D = device_instance(inst, G, st);
+const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st, uint32_t order) {
+  DeviceInstance *D = device_instance(inst, G, st, order);
   if (!D) return nullptr;
   std::lock_guard<std::mutex> lk(const_cast<tamc_instance *>(inst)->mu);
-  return D->sched[G].get();
+  return D->sched[2u * G + order].get();
 }
 
 }  // namespace tamc

@@ -33,6 +33,7 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
   for (uint32_t b = 1; b < P->num_betas; ++b)
     if (!(P->betas[b] - P->betas[b - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
   if (P->lo_beta_idx > P->num_betas) return fail(TAMC_E_INVALID_ARG, "lo_beta_idx out of range");
+  if (P->sweep_order > TAMC_ORDER_SEQUENTIAL) return fail(TAMC_E_INVALID_ARG, "unknown sweep_order");
   if ((P->sample_enabled && P->sample == 0) || (P->sample_states_enabled && P->sample_states == 0))
     return fail(TAMC_E_INVALID_ARG, "sample intervals must be positive");
   if (inst->n == 0) return fail(TAMC_E_INVALID_ARG, "empty instance");
@@ -131,9 +132,9 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
   else TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
   for (auto &g : S.groups) {
     tamc_status st;
-    DeviceInstance *D = device_instance(S.inst, g.G, &st);
+    DeviceInstance *D = device_instance(S.inst, g.G, &st, S.P.sweep_order);
     if (!D) return st;
-    const DeviceSchedule *Sc = device_schedule(S.inst, g.G, &st);
+    const DeviceSchedule *Sc = device_schedule(S.inst, g.G, &st, S.P.sweep_order);
     if (!Sc) return st;
     PtGroup &X = g.g;
     X.row_ptr = (const uint32_t *)D->row_ptr.p;
@@ -511,8 +512,9 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
 
 uint32_t default_waves(const tamc_instance *inst) {
   // enough waves to cover the typical level width, within one work group
-  const uint32_t nq = inst->n_groups;
-  const uint32_t width = inst->n_levels ? (nq + inst->n_levels - 1) / inst->n_levels : 1;
+  const SiteSchedule &H = inst->sched[kOrderColour];
+  const uint32_t nq = H.n_groups;
+  const uint32_t width = H.n_classes ? (nq + H.n_classes - 1) / H.n_classes : 1;
   uint32_t w = 4;
   while (w < width && w < 16) w *= 2;
   return w;
@@ -582,10 +584,10 @@ tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params,
 // discarded).  Choices are cached per batch shape for the life of the process.
 namespace {
 struct TuneKey {
-  uint32_t count, max_n, B, C;
+  uint32_t count, max_n, B, C, order;
   uint64_t sum_n;
   bool operator<(const TuneKey &o) const {
-    return std::tie(count, max_n, B, C, sum_n) < std::tie(o.count, o.max_n, o.B, o.C, o.sum_n);
+    return std::tie(count, max_n, B, C, order, sum_n) < std::tie(o.count, o.max_n, o.B, o.C, o.order, o.sum_n);
   }
 };
 std::mutex g_tune_mu;
@@ -594,8 +596,9 @@ std::map<TuneKey, uint32_t> g_tune_cache;
 
 static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                                   uint32_t *waves_out) {
-  TuneKey key{count, 0, 0, 0, 0};
+  TuneKey key{count, 0, 0, 0, 0, 0};
   for (uint32_t k = 0; k < count; ++k) {
+    key.order |= 1u << params[k].sweep_order;
     key.max_n = std::max(key.max_n, insts[k]->n);
     key.sum_n += insts[k]->n;
     key.B = std::max(key.B, params[k].num_betas);

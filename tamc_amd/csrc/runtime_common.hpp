@@ -80,6 +80,14 @@ struct Arena {
   }
 };
 
+enum : uint32_t { kOrderColour = 0, kOrderSequential = 1 };
+
+struct SiteSchedule {
+  std::vector<uint32_t> group_sites;   // [n_groups][4], kNone pads the last group of a class
+  std::vector<uint32_t> class_groups;  // [n_classes+1] offsets into groups
+  uint32_t n_classes = 0, n_groups = 0;
+};
+
 struct DeviceSchedule {
   DevBuf lvl_off, tasks, groups;
   uint32_t n_levels = 0, n_tasks = 0;
@@ -89,7 +97,7 @@ struct DeviceInstance {
   DevBuf row_ptr, col, J, bias, g_row_ptr, g_col, chi, ell_col, ell_col16, ell_J, self_J;
   bool has_self = false;
   uint32_t ell_D = 0;
-  std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by words per site G
+  std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by 2 * (words per site G) + order
 };
 
 }  // namespace tamc
@@ -103,19 +111,22 @@ struct tamc_instance {
   std::vector<uint32_t> g_row_ptr, g_col;  // coupling CsMat pattern (ICM graph)
   uint32_t g_nodes = 0;
   std::vector<std::vector<double>> chi;
-  // Sequential-order schedule: level(i) = 1 + max level of the lower-numbered sites coupled to i
-  // (0 if none); sites of one level are mutually uncoupled.  Within a level, sites in ascending
-  // order form groups of four ("RNG groups", numbered level-major); one Philox call keyed by the
-  // group serves its four sites.
-  std::vector<uint32_t> group_sites;  // [n_groups][4], kNone pads the last group of a level
-  std::vector<uint32_t> level_groups; // [n_levels+1] offsets into groups
-  uint32_t n_levels = 0, n_groups = 0;
+  // Sweep schedules: classes of mutually uncoupled sites, swept class by class.  Within a class,
+  // sites in ascending order form groups of four ("RNG groups", numbered class-major); one Philox
+  // call keyed by the group serves its four sites.
+  //   [kOrderColour]     greedy colouring in site order: colour(i) = the smallest colour not
+  //                      taken by a lower-numbered site coupled to i;
+  //   [kOrderSequential] levels: level(i) = 1 + max level of the lower-numbered sites coupled to i
+  //                      (0 if none), which reproduces the reference's sequential site order.
+  tamc::SiteSchedule sched[2];
   std::mutex mu;
   std::unique_ptr<tamc::DeviceInstance> dev;
 };
 
 namespace tamc {
-// Upload the instance (once) and the schedule for G words per site; returns nullptr on error.
-DeviceInstance *device_instance(const tamc_instance *inst, uint32_t G, tamc_status *st);
-const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st);
+// Upload the instance (once) and the schedule of the given order for G words per site; returns
+// nullptr on error.
+DeviceInstance *device_instance(const tamc_instance *inst, uint32_t G, tamc_status *st, uint32_t order = kOrderColour);
+const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st,
+                                      uint32_t order = kOrderColour);
 }  // namespace tamc

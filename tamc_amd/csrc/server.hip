@@ -206,7 +206,7 @@ struct tamc_server {
   // task latency per (4 sites x 64 replicas) / waves, plus the ICM passes over the pairs.
   static double model_us_per_sweep(const Request &r, uint32_t gi) {
     const GroupPlan &g = r.S.groups[gi];
-    const double tasks = (double)r.S.inst->n_groups * g.G;
+    const double tasks = (double)r.S.inst->sched[r.S.P.sweep_order].n_groups * g.G;
     return 8.0 + 5.0 * tasks / r.waves * (g.P ? 1.8 : 1.0);
   }
 
@@ -543,8 +543,8 @@ tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, cons
   r->S.init_states = nullptr;  // copied into the groups' init_bits
   // the instance is uploaded (once per instance) here, in the caller's thread, not in the loop
   for (auto &g : r->S.groups) {
-    if (!device_instance(inst, g.G, &s)) return s;
-    if (!device_schedule(inst, g.G, &s)) return s;
+    if (!device_instance(inst, g.G, &s, r->S.P.sweep_order)) return s;
+    if (!device_schedule(inst, g.G, &s, r->S.P.sweep_order)) return s;
   }
   for (auto &g : r->S.groups) r->lds = std::max(r->lds, lds_bytes_for(r->S, g, r->waves));
   r->lds_class = tamc_server::lds_class_of(r->lds);

@@ -77,16 +77,17 @@ def test_houdayer_flip_sets(gpu, kind, kw):
 
 
 def _run_both(kind, kw, B=8, C=2, T=60, lo=4, threads=1, seed=5, icm=True, sample=4, sstates=4, limiting=0,
-              bmin=0.2, bmax=5.0):
+              bmin=0.2, bmax=5.0, order="colour"):
     oi, gi, _ = make_pair(kind, seed=seed, **kw)
     params = api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(bmin, bmax, B),
                              lo_num_beta=B - lo, icm=icm, num_replica_chains=C, threads=threads, sample=sample,
-                             sample_states=sstates, sample_limiting=limiting, seed=seed * 1000 + 17)
+                             sample_states=sstates, sample_limiting=limiting, seed=seed * 1000 + 17,
+                             sweep_order=order)
     runner = api.PtIcmRunner(gi, params)
     g = runner.run()
     o = O.pt_run(oi, runner.betas, T, num_chains=C, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=icm,
                  threaded=threads > 1, sample=sample, sample_states=sstates, sample_limiting=limiting,
-                 rng_mode=O.RNG_PHILOX, seed=params.seed)
+                 rng_mode=O.RNG_PHILOX, seed=params.seed, sweep_order=order)
     return g, o
 
 
@@ -95,6 +96,15 @@ def _run_both(kind, kw, B=8, C=2, T=60, lo=4, threads=1, seed=5, icm=True, sampl
     ("rr3_pm", dict(n=64), 2, 1, 0), ("qubo", dict(n=24, density=0.2), 2, 1, 0), ("2d_pm", dict(L=5), 1, 1, 0)])
 def test_pt_production_exact(gpu, kind, kw, C, threads, limiting):
     g, o = _run_both(kind, kw, C=C, threads=threads, limiting=limiting)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=True)
+
+
+@pytest.mark.parametrize("kind,kw,C,threads", [("2d_pm", {}, 2, 1), ("rr3_pm", dict(n=64), 2, 4),
+                                               ("qubo", dict(n=24, density=0.2), 3, 1), ("3d_pm", dict(L=3), 2, 1)])
+def test_pt_production_sequential_order(gpu, kind, kw, C, threads):
+    # level schedule: the reference's own sequential site order
+    g, o = _run_both(kind, kw, C=C, threads=threads, order="sequential")
     assert o["error"] == 0
     assert_same(g, o, exact_energy=True)
 
