@@ -446,69 +446,133 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
 }
 
 // 2. replica exchange (tamc-core/src/pt.rs:126-144; threaded variant tamc-core/src/parallel/pt.rs:202-215)
-//    and round trips (tamc-core/src/pt.rs:54-84).  Energy differences and uniforms are prepared in
-//    parallel; one lane per chain then walks the ladder carrying only registers.
+//    and round trips (tamc-core/src/pt.rs:54-84).  One wave per chain, lane j = ladder step j
+//    (slots j, j+1).  The threaded variant decides every step independently; the serial one
+//    carries dE into the next step after a swap, so its decisions are a scan over the steps,
+//    done on values read from the lanes' registers.  The walk's permutation then follows from the
+//    decisions in parallel: when step i rejects, the walker restarts at slot i+1, so slot j
+//    receives slot j+1 on acceptance and otherwise the slot where the walk last restarted.
+__device__ __forceinline__ void exchange_serial_walk(Ctx &c, uint32_t t, uint32_t ch) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t B = c.B, B1 = B - 1u;
+  uint32_t *perm = c.perm + ch * B;
+  uint8_t *tags = c.tags + ch * B;
+  const bool threaded = (c.flags & kFlagThreaded) != 0u;
+  uint32_t cur = perm[0];
+  uint8_t curtag = tags[0];
+  bool prev_ok = false;
+  float prev = 0.0f;
+  for (uint32_t j = 0; j < B1; ++j) {
+    const float dEj = c.energy[perm[j + 1]] - c.energy[perm[j]];
+    const U4 v = philox4x32_10(t, j >> 2, X.chain0 + ch, kTagExchange, c.key0, c.key1);
+    // serial variant: after a swap at j-1, dE[j] += dE[j-1] (the value used at j-1)
+    const float eff = (!threaded && prev_ok) ? dEj + prev : dEj;
+    const float dlt = (c4(X.betas)[j + 1] - c4(X.betas)[j]) * eff;
+    bool ok = dlt >= 0.0f;
+    if (!ok) ok = accept_u(dlt, pick(v, j));
+    const uint32_t nxt = perm[j + 1];
+    const uint8_t nxttag = tags[j + 1];
+    if (ok) {
+      perm[j] = nxt;
+      tags[j] = nxttag;
+      c.nacc[ch * B + j] += 1u;
+    } else {
+      perm[j] = cur;
+      tags[j] = curtag;
+      cur = nxt;
+      curtag = nxttag;
+    }
+    prev_ok = ok;
+    prev = eff;
+  }
+  perm[B1] = cur;
+  tags[B1] = curtag;
+}
+
+__device__ __forceinline__ void round_trips(Ctx &c, uint32_t ch) {
+  // update_round_trips; tags: 0 None 1 Min 2 Max 3 MinRefl 4 MaxRefl
+  const uint32_t B = c.B, B1 = B - 1u;
+  uint8_t *tags = c.tags + ch * B;
+  uint8_t lo = tags[0];
+  if (lo == 0) lo = 1;
+  else if (lo == 4) {
+    c.rt[ch] += 1u;
+    lo = 1;
+  } else if (lo == 2) lo = 3;
+  tags[0] = lo;
+  uint8_t hi = tags[B1];
+  if (hi == 0) hi = 2;
+  else if (hi == 3) {
+    c.rt[ch] += 1u;
+    hi = 2;
+  } else if (hi == 1) hi = 4;
+  tags[B1] = hi;
+}
+
 __device__ __forceinline__ void exchange_phase(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t B = c.B, B1 = B - 1u;
-  for (uint32_t k = c.tid; k < c.nch * B1; k += c.nthreads) {
-    const uint32_t ch = k / B1, j = k - ch * B1;
-    const uint32_t *perm = c.perm + ch * B;
-    c.dEx[ch * B + j] = c.energy[perm[j + 1]] - c.energy[perm[j]];
-    const U4 v = philox4x32_10(t, j >> 2, X.chain0 + ch, kTagExchange, c.key0, c.key1);
-    c.uw[ch * B + j] = pick(v, j);
-  }
-  __syncthreads();
-  if (c.wave == 0 && c.lane < c.nch) {
-    const uint32_t ch = c.lane;
+  const bool threaded = (c.flags & kFlagThreaded) != 0u;
+  for (uint32_t ch = c.wave; ch < c.nch; ch += c.W) {
+    if (B > 64u) {  // ladders longer than a wave: one lane walks them
+      if (c.lane == 0) {
+        exchange_serial_walk(c, t, ch);
+        round_trips(c, ch);
+      }
+      continue;
+    }
     uint32_t *perm = c.perm + ch * B;
     uint8_t *tags = c.tags + ch * B;
-    const float *dE = c.dEx + ch * B;
-    const uint32_t *uw = c.uw + ch * B;
-    const bool threaded = (c.flags & kFlagThreaded) != 0u;
-    uint32_t cur = perm[0];
-    uint8_t curtag = tags[0];
-    bool prev_ok = false;
-    float prev = 0.0f;
-    for (uint32_t j = 0; j < B1; ++j) {
-      // serial variant: after a swap at j-1, dE[j] += dE[j-1] (the value used at j-1)
-      const float eff = (!threaded && prev_ok) ? dE[j] + prev : dE[j];
-      const float dbeta = c4(X.betas)[j + 1] - c4(X.betas)[j];
-      const float dlt = dbeta * eff;
-      bool ok = dlt >= 0.0f;
-      if (!ok) ok = accept_u(dlt, uw[j]);
-      const uint32_t nxt = perm[j + 1];
-      const uint8_t nxttag = tags[j + 1];
-      if (ok) {
-        perm[j] = nxt;
-        tags[j] = nxttag;
-        c.nacc[ch * B + j] += 1u;
-      } else {
-        perm[j] = cur;
-        tags[j] = curtag;
-        cur = nxt;
-        curtag = nxttag;
-      }
-      prev_ok = ok;
-      prev = eff;
+    const uint32_t j = c.lane;
+    const bool step = j < B1;
+    float dE = 0.f, dbeta = 0.f;
+    uint32_t w = 0u;
+    if (step) {
+      dE = c.energy[perm[j + 1]] - c.energy[perm[j]];
+      dbeta = c4(X.betas)[j + 1] - c4(X.betas)[j];
+      const U4 v = philox4x32_10(t, j >> 2, X.chain0 + ch, kTagExchange, c.key0, c.key1);
+      w = pick(v, j);
     }
-    perm[B1] = cur;
-    tags[B1] = curtag;
-    // update_round_trips; tags: 0 None 1 Min 2 Max 3 MinRefl 4 MaxRefl
-    uint8_t lo = tags[0];
-    if (lo == 0) lo = 1;
-    else if (lo == 4) {
-      c.rt[ch] += 1u;
-      lo = 1;
-    } else if (lo == 2) lo = 3;
-    tags[0] = lo;
-    uint8_t hi = tags[B1];
-    if (hi == 0) hi = 2;
-    else if (hi == 3) {
-      c.rt[ch] += 1u;
-      hi = 2;
-    } else if (hi == 1) hi = 4;
-    tags[B1] = hi;
+    uint64_t okmask;
+    if (threaded) {
+      const float dlt = dbeta * dE;
+      const bool ok = step && (dlt >= 0.0f || accept_u(dlt, w));
+      okmask = ballot(ok);
+    } else {
+      okmask = 0ull;
+      bool prev_ok = false;
+      float prev = 0.0f;
+      for (uint32_t k = 0; k < B1; ++k) {
+        const float dEk = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(dE), k));
+        const float dbk = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(dbeta), k));
+        const uint32_t wk = __builtin_amdgcn_readlane(w, k);
+        // after a swap at k-1, dE[k] += dE[k-1] (the value used at k-1)
+        const float eff = prev_ok ? dEk + prev : dEk;
+        const float dlt = dbk * eff;
+        const bool ok = dlt >= 0.0f || accept_u(dlt, wk);
+        if (ok) okmask |= 1ull << k;
+        prev_ok = ok;
+        prev = eff;
+      }
+    }
+    // slot j after the walk: j+1 if step j accepted, else the last restart point r(j)
+    const uint64_t rejected = ~okmask & ((B1 >= 64u) ? ~0ull : ((1ull << B1) - 1ull));
+    const uint64_t below = rejected & ((j >= 64u) ? ~0ull : ((1ull << j) - 1ull));
+    const uint32_t r = below ? 64u - (uint32_t)__builtin_clzll(below) : 0u;
+    const bool okj = step && ((okmask >> j) & 1ull);
+    uint32_t np = 0u;
+    uint8_t nt = 0;
+    if (j < B) {
+      const uint32_t src = okj ? j + 1u : r;
+      np = perm[src];
+      nt = tags[src];
+    }
+    if (j < B) {  // every read above is done before any write (one wave, LDS in order)
+      perm[j] = np;
+      tags[j] = nt;
+      if (okj) c.nacc[ch * B + j] += 1u;
+    }
+    if (c.lane == 0) round_trips(c, ch);
   }
   __syncthreads();
   for (uint32_t k = c.tid; k < c.nch * B; k += c.nthreads) {

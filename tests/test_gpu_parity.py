@@ -131,6 +131,13 @@ def test_pt_large_ladder_two_words(gpu):
     assert_same(g, o, exact_energy=True)
 
 
+@pytest.mark.parametrize("threads", [1, 2])
+def test_pt_ladder_longer_than_a_wave(gpu, threads):
+    # 80 betas: the exchange walk no longer fits one lane per step of a wave; 160 replicas, 3 words
+    g, o = _run_both("2d_pm", dict(L=6), B=80, lo=60, T=20, threads=threads, sample=3, sstates=5)
+    assert_same(g, o, exact_energy=True)
+
+
 def _streams(count, length, seed):
     x = O.Xoshiro(seed)
     out = []
