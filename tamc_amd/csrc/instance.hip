@@ -234,8 +234,11 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
         ec[(size_t)i * Dl + k] = real ? inst->col[e] : i;
         eJ[(size_t)i * Dl + k] = real ? inst->J[e] : 0.0f;
       }
-    std::vector<uint16_t> ec16(ec.size());
-    for (size_t k = 0; k < ec.size(); ++k) ec16[k] = (uint16_t)ec[k];
+    // 16-bit lists with rows padded to a multiple of 4 entries (padding = the row's own site)
+    const uint32_t Ds = nbr_stride(Dl);
+    std::vector<uint16_t> ec16((size_t)inst->n * std::max(Ds, 1u));
+    for (uint32_t i = 0; i < inst->n && Dl; ++i)
+      for (uint32_t k = 0; k < Ds; ++k) ec16[(size_t)i * Ds + k] = (uint16_t)(k < Dl ? ec[(size_t)i * Dl + k] : i);
     if (!up(D->ell_col, ec.data(), ec.size() * 4) || !up(D->ell_J, eJ.data(), eJ.size() * 4) ||
         (inst->n < 65536u && !up(D->ell_col16, ec16.data(), ec16.size() * 2))) {
       *st = fail(TAMC_E_DEVICE, "instance upload failed");

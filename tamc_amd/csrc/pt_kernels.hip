@@ -239,7 +239,7 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
         float nJ[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          nj[k] = M.nbr((size_t)i * D + k);
+          nj[k] = M.nbr((size_t)i * nbr_stride(D) + k);
           nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
         }
         uint32_t out[D], sj[D];
@@ -390,13 +390,30 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     __syncthreads();
     ticks[kPhIcmPasses] += 1u;
     int changed = 0;
+    // two rounds over the sites between barriers: updates made by other waves during the first
+    // round are already visible in the second, so clusters grow by more than one hop per barrier
+    for (uint32_t rep = 0; rep < 2u; ++rep)
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
       const uint32_t q = M.qw[i];
       const uint32_t cur = M.cw[i];
       if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
       uint32_t nb = 0;
-      if (D) {
-        for (uint32_t k = 0; k < D; ++k) nb |= M.cw[M.nbr((size_t)i * D + k)];
+      if (D != 0u && D <= 4u) {
+        // one 8-byte load of the (padded) row
+        const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
+        nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu] | M.cw[w >> 48];
+      } else if (D != 0u) {
+        uint64_t w0, w1;
+        if (M.nbr_lds) {
+          w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i];
+          w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i + 1u];
+        } else {
+          w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i];
+          w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i + 1u];
+        }
+        nb = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu] | M.cw[w0 >> 48] |
+             M.cw[w1 & 0xFFFFu] | M.cw[(w1 >> 16) & 0xFFFFu] | M.cw[(w1 >> 32) & 0xFFFFu] | M.cw[w1 >> 48];
       } else if (i < X.g_nodes) {
         const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
         for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
@@ -903,7 +920,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   if (c.P > 0 && X.ell_col16 != nullptr && X.ell_D != 0u) {
     if (L.nbr != kNone) {
       uint16_t *nb = (uint16_t *)(smem + L.nbr);
-      for (uint32_t k = c.tid; k < c.n * X.ell_D; k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
+      for (uint32_t k = c.tid; k < c.n * nbr_stride(X.ell_D); k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
       c.nbr = nb;
       c.nbr_in_lds = true;
     } else {

@@ -80,7 +80,7 @@ struct PtGroup {
   const uint32_t *icm_k;
   // ---- fixed-stride copy of coupling_vecs for rows of at most ell_D entries (0 = use CSR) ----
   const uint32_t *ell_col;   // [n][ell_D], padding entries point at the row's own site
-  const uint16_t *ell_col16; // the same as 16-bit indices (n < 65536), cached in LDS for ICM
+  const uint16_t *ell_col16; // 16-bit neighbour lists [n][nbr_stride(ell_D)] (n < 65536), for ICM
   const float *ell_J;        // [n][ell_D], padding entries are +0.0
   uint32_t ell_D, pad2;
   // ---- QUBO self entries (src/ising.rs:308-310): J_ii per site, null when the instance has none.
@@ -160,9 +160,13 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
 }
 
 // Adds the LDS copy of the 16-bit neighbour lists used by ICM when it fits under `limit`.
+// Row stride of the 16-bit neighbour lists: D rounded up to 4, so that a row is one 8- or 16-byte
+// load (padding entries repeat the row's own site).
+__host__ __device__ inline uint32_t nbr_stride(uint32_t D) { return (D + 3u) & ~3u; }
+
 __host__ __device__ inline LdsLayout with_nbr(LdsLayout L, uint32_t n, uint32_t D, uint32_t limit) {
   if (D == 0u || L.qw == kNone) return L;
-  const uint32_t need = lds_align(n * D * 2u);
+  const uint32_t need = lds_align(n * nbr_stride(D) * 2u);
   if (L.total + need > limit) return L;
   L.nbr = L.total;
   L.total += need;
