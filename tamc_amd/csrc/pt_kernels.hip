@@ -237,11 +237,17 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       if constexpr (D > 0) {
         uint32_t nj[D];
         float nJ[D];
+        if constexpr (D <= 4) {
+          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          nj[k] = M.nbr((size_t)i * nbr_stride(D) + k);
-          nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
+          for (int k = 0; k < D; ++k) nj[k] = (uint32_t)(w >> (16 * k)) & 0xFFFFu;
+        } else {
+#pragma unroll
+          for (int k = 0; k < D; ++k) nj[k] = M.nbr((size_t)i * nbr_stride(D) + k);
         }
+#pragma unroll
+        for (int k = 0; k < D; ++k) nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
         uint32_t out[D], sj[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
