@@ -270,8 +270,28 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
     }
     S->n_levels = H.n_classes;
     S->n_tasks = (uint32_t)tasks.size();
+    const uint32_t Dl = D->ell_D, RS = 8u + 8u * Dl;
+    std::vector<uint32_t> rec((size_t)H.n_groups * RS, 0u);
+    for (uint32_t q = 0; q < H.n_groups; ++q) {
+      uint32_t *r = rec.data() + (size_t)q * RS;
+      for (uint32_t s = 0; s < 4; ++s) {
+        const uint32_t i = H.group_sites[4 * q + s];
+        r[s] = i;
+        if (i == kNone) continue;
+        memcpy(&r[4 + s], &inst->bias[i], 4);
+        for (uint32_t k = 0; k < Dl; ++k) {
+          const uint32_t e = inst->row_ptr[i] + k;
+          const bool real = e < inst->row_ptr[i + 1];
+          r[8 + s * Dl + k] = real ? inst->col[e] : i;
+          const float Jv = real ? inst->J[e] : 0.0f;
+          memcpy(&r[8 + 4 * Dl + s * Dl + k], &Jv, 4);
+        }
+      }
+    }
+    S->record_words = RS;
     if (!up(S->lvl_off, cnt.data(), cnt.size() * 4) || !up(S->tasks, tasks.data(), tasks.size() * 4) ||
-        !up(S->groups, H.group_sites.data(), H.group_sites.size() * 4)) {
+        !up(S->groups, H.group_sites.data(), H.group_sites.size() * 4) ||
+        !up(S->records, rec.data(), rec.size() * 4)) {
       *st = fail(TAMC_E_DEVICE, "schedule upload failed");
       return nullptr;
     }

@@ -666,19 +666,42 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       const uint32_t rho = X.chain0 * c.B + r;
       const U4 u = philox4x32_10(t, grp, rho, kTagSweep, c.key0, c.key1);
       uint32_t site[4];
-#pragma unroll
-      for (uint32_t s = 0; s < 4u; ++s) site[s] = c4(X.grp)[grp * 4u + s];
       // the four sites of a group are mutually uncoupled: gather all fields first (independent
       // LDS reads in flight together), then decide and write the new spin words
       float d[4];
       uint32_t bi[4];
+      if constexpr (D > 0) {
+        // the group's record: sites, biases, cols, couplings, in one contiguous block
+        const TAMC_AS4 uint32_t *rec = c4(X.records) + (size_t)grp * (8u + 8u * D);
+        const uint32_t up = 31u - sh;
 #pragma unroll
-      for (uint32_t s = 0; s < 4u; ++s) {
-        d[s] = 0.0f;
-        bi[s] = 0u;
-        if (site[s] != kNone) {
-          d[s] = site_field<D>(c, site[s], g, half, sh);
-          bi[s] = (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u;
+        for (uint32_t s = 0; s < 4u; ++s) site[s] = rec[s];
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s) {
+          // padded sites have record zeros: a harmless field of site 0
+          uint32_t w[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) w[k] = c.sp[(rec[8u + s * D + k] * G + g) * 2u + half];
+          float f = 0.0f;
+          f += __uint_as_float(rec[4u + s]);
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            f = __fmaf_rn(__uint_as_float(rec[8u + 4u * D + s * D + k]),
+                          __uint_as_float(0x3f800000u | ((w[k] << up) & 0x80000000u)), f);
+          d[s] = f;
+          bi[s] = (site[s] != kNone) ? (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u : 0u;
+        }
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s) site[s] = c4(X.grp)[grp * 4u + s];
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s) {
+          d[s] = 0.0f;
+          bi[s] = 0u;
+          if (site[s] != kNone) {
+            d[s] = site_field<D>(c, site[s], g, half, sh);
+            bi[s] = (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u;
+          }
         }
       }
       float dd[4], xs[4];

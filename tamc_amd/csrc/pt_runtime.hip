@@ -158,6 +158,7 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
     X.tasks = (const uint32_t *)Sc->tasks.p;
     X.n_levels = Sc->n_levels;
     X.grp = (const uint32_t *)Sc->groups.p;
+    X.records = (const uint32_t *)Sc->records.p;
     X.betas = S.d_betas;
     X.B = S.B;
     X.nch = g.nch;

@@ -46,6 +46,7 @@ struct PtGroup {
   const uint32_t *lvl_off;   // [n_levels+1] offsets into tasks
   const uint32_t *tasks;     // group*G + g
   const uint32_t *grp;       // [n_groups][4] sites of each group (kNone pads)
+  const uint32_t *records;   // [n_groups][8 + 8 * ell_D]: sites, bias, cols, J of each group
   uint32_t n_levels;
   // ---- ladder ----
   const float *betas;        // [B]

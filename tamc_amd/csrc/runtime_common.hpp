@@ -90,7 +90,11 @@ struct SiteSchedule {
 
 struct DeviceSchedule {
   DevBuf lvl_off, tasks, groups;
-  uint32_t n_levels = 0, n_tasks = 0;
+  // per RNG group, everything its sweep task reads, contiguous so that one task loads it with a
+  // few wide scalar loads: sites[4], bias bits[4], then for fixed-stride rows (D > 0)
+  // cols[4][D] and J bits[4][D]; padded sites have zeros
+  DevBuf records;
+  uint32_t n_levels = 0, n_tasks = 0, record_words = 0;
 };
 
 struct DeviceInstance {
