@@ -182,27 +182,38 @@ struct SiteWords {
   }
 };
 
-// OR over the lanes of each group of GS consecutive lanes (GS = 16 or 32) with DPP row shifts
-// (and a row broadcast for 32); the result is complete in the group's last lane.  Every lane of
+// Inclusive prefix sum over the 64 lanes with DPP row shifts and row broadcasts.  Every lane of
 // the wave must be active.
-template <int GS>
-__device__ __forceinline__ uint32_t group_or(uint32_t v) {
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  if constexpr (GS == 32) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
 }
 
-// ICM step (d): swap the cluster spins of both replicas of every pair (= flip both).  Lanes are
-// (site slot, pair): a wave takes 64/GS sites per step and lane (s, p) handles pair p at site i_s.
-// When i_s is in cluster p the lane adds the first replica's energy change
-//   -2 s_a(i) (h_i + sum_{j not in cluster p} J_ij s_a(j))
-// to its own fixed-point accumulator; the second replica's change is its negative (the move
-// conserves E_a + E_b).  The flip masks of a site are ORed over its group of lanes and written by
-// the group's last lane.  Pair p only reads replica a_p at neighbours outside cluster p and only
-// flips replicas a_p, b_p at sites inside it, so sites are flipped as soon as they are done.
+// value of v in lane `src` (0..63); every lane of the wave must be active
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
+}
+
+// index of the r-th set bit (r < popcount(x)) of x
+__device__ __forceinline__ uint32_t select_bit(uint32_t x, uint32_t r) {
+  uint32_t p = 0, c;
+  c = (uint32_t)__popc(x & 0xFFFFu);
+  if (r >= c) { r -= c; x >>= 16; p += 16; }
+  c = (uint32_t)__popc(x & 0xFFu);
+  if (r >= c) { r -= c; x >>= 8; p += 8; }
+  c = (uint32_t)__popc(x & 0xFu);
+  if (r >= c) { r -= c; x >>= 4; p += 4; }
+  c = (uint32_t)__popc(x & 0x3u);
+  if (r >= c) { r -= c; x >>= 2; p += 2; }
+  if (r >= (x & 1u)) p += 1;
+  return p;
+}
+
 // ICM scratch -- overlap words qw, cluster words cw, per-block counts blk -- and the u16
 // neighbour lists, with address spaces fixed at compile time: the scratch is in LDS (kLds) or in
 // HBM for groups too large for LDS, and the neighbour lists are an LDS copy or the HBM table.
@@ -217,22 +228,41 @@ struct IcmMem {
   __device__ __forceinline__ uint32_t nbr(size_t k) const { return nbr_lds ? nbr_lds[k] : nbr_glb[k]; }
 };
 
-template <int D, int GS, bool kLds>
+// ICM step (d): swap the cluster spins of both replicas of every pair (= flip both) and sum the
+// first replica's energy change
+//   dE_a(p) = sum_{i in cluster p} -2 s_a(i) (h_i + sum_{j not in cluster p} J_ij s_a(j));
+// the second replica's change is -dE_a (the move conserves E_a + E_b).  A wave takes 64 sites at
+// a time; their (site, pair) memberships are compacted over the lanes (prefix sum of the cluster
+// words' popcounts), so every lane computes one term of a cluster per round, and the terms are
+// added into the pair's fixed-point total.  Then lane = site applies the site's flip mask.  Pair p
+// only reads replica a_p at neighbours outside cluster p and only flips replicas a_p, b_p at sites
+// inside it, so a block is flipped as soon as its terms are done.
+template <int D, bool kLds>
 __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t P = c.P, n = c.n, G = c.G;
-  constexpr uint32_t SPW = 64u / GS;
-  const uint32_t p = c.lane % GS, slot = c.lane / GS;
-  const bool pv = p < P;
-  const uint32_t a = pv ? c.pair_a[p] : 0u, b = pv ? c.pair_b[p] : 0u;
-  long long acc = 0;
-  const uint32_t steps = (n + SPW - 1u) / SPW;
-  for (uint32_t st = c.wave; st < steps; st += c.W) {
-    const uint32_t i = st * SPW + slot;
-    const uint32_t cwi = (i < n) ? M.cw[i] : 0u;
-    if (ballot(cwi != 0u) == 0ull) continue;
-    const bool in = pv && ((cwi >> p) & 1u);
-    if (in) {
+  // replica indices of every pair, lane p holding pair p
+  const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
+  const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
+  // one round: lane L takes the (base+L)-th (site, pair) membership of block blk
+  auto round = [&](uint32_t blk, uint32_t cwi, uint32_t excl, uint32_t total, uint32_t base) {
+    const uint32_t m = base + c.lane;
+    // the item's site: the last lane whose range starts at or before m
+    uint32_t s = 0, start = 0;
+#pragma unroll
+    for (uint32_t step = 32u; step >= 1u; step >>= 1) {
+      const uint32_t probe = s + step;
+      const uint32_t v = lane_value(excl, probe & 63u);
+      if (probe < 64u && v <= m) {
+        s = probe;
+        start = v;
+      }
+    }
+    const uint32_t cws = lane_value(cwi, s);
+    if (m < total) {
+      const uint32_t p = select_bit(cws, m - start);
+      const uint32_t a = c.pair_a[p];
+      const uint32_t i = blk * 64u + s;
       float hh = c4(X.bias)[i];
       if constexpr (D > 0) {
         uint32_t nj[D];
@@ -268,29 +298,56 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
         }
       }
       const float term = sbit(c.sp, G, i, a) ? 2.0f * hh : -2.0f * hh;
-      acc += to_fix(term);
+      atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)to_fix(term));
     }
+  };
+  // lane = site: flip replicas a_p, b_p for every pair p whose cluster holds the site
+  auto flip_block = [&](uint32_t blk, uint32_t cwi) {
+    const uint32_t i_lane = blk * 64u + c.lane;
     for (uint32_t g = 0; g < G; ++g) {
-      uint32_t lo = 0u, hi = 0u;
-      if (in) {
-        if ((a >> 6) == g) {
-          if (a & 32u) hi |= 1u << (a & 31u);
-          else lo |= 1u << (a & 31u);
-        }
-        if ((b >> 6) == g) {
-          if (b & 32u) hi |= 1u << (b & 31u);
-          else lo |= 1u << (b & 31u);
-        }
+      uint64_t mg = 0ull;
+      for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t a = __builtin_amdgcn_readlane(pa_l, p), b = __builtin_amdgcn_readlane(pb_l, p);
+        const uint64_t bits = ((a >> 6) == g ? 1ull << (a & 63u) : 0ull) | ((b >> 6) == g ? 1ull << (b & 63u) : 0ull);
+        if ((cwi >> p) & 1u) mg |= bits;
       }
-      lo = group_or<GS>(lo);
-      hi = group_or<GS>(hi);
-      if (p == GS - 1u && i < n) {
-        if (lo) c.sp[(i * G + g) * 2u] ^= lo;
-        if (hi) c.sp[(i * G + g) * 2u + 1u] ^= hi;
+      if (mg) {
+        c.sp[(i_lane * G + g) * 2u] ^= (uint32_t)mg;
+        c.sp[(i_lane * G + g) * 2u + 1u] ^= (uint32_t)(mg >> 32);
       }
+    }
+  };
+  if (c.nblk >= c.W) {
+    // a wave owns whole blocks: terms, then flips, block after block
+    for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+      const uint32_t i_lane = blk * 64u + c.lane;
+      const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
+      const uint32_t cnt = (uint32_t)__popc(cwi);
+      const uint32_t incl = wave_incl_scan(cnt);
+      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+      for (uint32_t base = 0; base < total; base += 64u) round(blk, cwi, incl - cnt, total, base);
+      if (total) flip_block(blk, cwi);
+    }
+  } else {
+    // fewer blocks than waves: every wave scans every block and the rounds go round-robin over
+    // the waves; the flips wait until every round is done
+    uint32_t unit = 0;
+    for (uint32_t blk = 0; blk < c.nblk; ++blk) {
+      const uint32_t i_lane = blk * 64u + c.lane;
+      const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
+      const uint32_t cnt = (uint32_t)__popc(cwi);
+      const uint32_t incl = wave_incl_scan(cnt);
+      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+      for (uint32_t base = 0; base < total; base += 64u, ++unit)
+        if (unit % c.W == c.wave) round(blk, cwi, incl - cnt, total, base);
+    }
+    __syncthreads();
+    for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+      const uint32_t i_lane = blk * 64u + c.lane;
+      const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
+      if (ballot(cwi != 0u)) flip_block(blk, cwi);
     }
   }
-  if (pv && acc != 0) atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)acc);
 }
 
 // 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
@@ -438,22 +495,12 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     tk = now;
   }
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
-  if (P <= 16u) {
-    switch (X.ell_D) {
-      case 3u: icm_flip<3, 16>(c, M); break;
-      case 4u: icm_flip<4, 16>(c, M); break;
-      case 6u: icm_flip<6, 16>(c, M); break;
-      case 8u: icm_flip<8, 16>(c, M); break;
-      default: icm_flip<0, 16>(c, M);
-    }
-  } else {
-    switch (X.ell_D) {
-      case 3u: icm_flip<3, 32>(c, M); break;
-      case 4u: icm_flip<4, 32>(c, M); break;
-      case 6u: icm_flip<6, 32>(c, M); break;
-      case 8u: icm_flip<8, 32>(c, M); break;
-      default: icm_flip<0, 32>(c, M);
-    }
+  switch (X.ell_D) {
+    case 3u: icm_flip<3>(c, M); break;
+    case 4u: icm_flip<4>(c, M); break;
+    case 6u: icm_flip<6>(c, M); break;
+    case 8u: icm_flip<8>(c, M); break;
+    default: icm_flip<0>(c, M);
   }
   __syncthreads();
   // moved pairs: energies become energy_ref of the new states, as the reference recomputes them
