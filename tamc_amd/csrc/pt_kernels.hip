@@ -245,7 +245,29 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
   const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
   // one round: lane L takes the (base+L)-th (site, pair) membership of block blk
-  auto round = [&](uint32_t blk, uint32_t cwi, uint32_t excl, uint32_t total, uint32_t base) {
+  // per-site data of a block, lane = site, loaded once per block and handed to the item lanes
+  struct SiteData {
+    uint32_t bias, nbr_lo, nbr_hi, J[D > 0 ? D : 1];
+  };
+  auto site_data = [&](uint32_t blk) {
+    SiteData sd{};
+    const uint32_t i = blk * 64u + c.lane;
+    if (i < n) {
+      sd.bias = __float_as_uint(c4(X.bias)[i]);
+      if constexpr (D > 0 && D <= 4) {
+        const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
+        sd.nbr_lo = (uint32_t)w;
+        sd.nbr_hi = (uint32_t)(w >> 32);
+      }
+      if constexpr (D > 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) sd.J[k] = __float_as_uint(c4(X.ell_J)[(size_t)i * D + k]);
+      }
+    }
+    return sd;
+  };
+  auto round = [&](uint32_t blk, uint32_t cwi, uint32_t excl, uint32_t total, uint32_t base, const SiteData &sd) {
     const uint32_t m = base + c.lane;
     // the item's site: the last lane whose range starts at or before m
     uint32_t s = 0, start = 0;
@@ -259,25 +281,32 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       }
     }
     const uint32_t cws = lane_value(cwi, s);
+    const float bias_s = __uint_as_float(lane_value(sd.bias, s));
+    uint32_t nlo = 0u, nhi = 0u;
+    float nJ[D > 0 ? D : 1];
+    if constexpr (D > 0) {
+      if constexpr (D <= 4) {
+        nlo = lane_value(sd.nbr_lo, s);
+        nhi = lane_value(sd.nbr_hi, s);
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) nJ[k] = __uint_as_float(lane_value(sd.J[k], s));
+    }
     if (m < total) {
       const uint32_t p = select_bit(cws, m - start);
       const uint32_t a = c.pair_a[p];
       const uint32_t i = blk * 64u + s;
-      float hh = c4(X.bias)[i];
+      float hh = bias_s;
       if constexpr (D > 0) {
         uint32_t nj[D];
-        float nJ[D];
         if constexpr (D <= 4) {
-          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
-                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
+          const uint64_t w = ((uint64_t)nhi << 32) | nlo;
 #pragma unroll
           for (int k = 0; k < D; ++k) nj[k] = (uint32_t)(w >> (16 * k)) & 0xFFFFu;
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) nj[k] = M.nbr((size_t)i * nbr_stride(D) + k);
         }
-#pragma unroll
-        for (int k = 0; k < D; ++k) nJ[k] = c4(X.ell_J)[(size_t)i * D + k];
         uint32_t out[D], sj[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
@@ -325,8 +354,10 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       const uint32_t cnt = (uint32_t)__popc(cwi);
       const uint32_t incl = wave_incl_scan(cnt);
       const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-      for (uint32_t base = 0; base < total; base += 64u) round(blk, cwi, incl - cnt, total, base);
-      if (total) flip_block(blk, cwi);
+      if (total == 0u) continue;
+      const SiteData sd = site_data(blk);
+      for (uint32_t base = 0; base < total; base += 64u) round(blk, cwi, incl - cnt, total, base, sd);
+      flip_block(blk, cwi);
     }
   } else {
     // fewer blocks than waves: every wave scans every block and the rounds go round-robin over
@@ -338,8 +369,10 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       const uint32_t cnt = (uint32_t)__popc(cwi);
       const uint32_t incl = wave_incl_scan(cnt);
       const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
+      if (total == 0u) continue;
+      const SiteData sd = site_data(blk);
       for (uint32_t base = 0; base < total; base += 64u, ++unit)
-        if (unit % c.W == c.wave) round(blk, cwi, incl - cnt, total, base);
+        if (unit % c.W == c.wave) round(blk, cwi, incl - cnt, total, base, sd);
     }
     __syncthreads();
     for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
