@@ -53,8 +53,13 @@ __device__ __forceinline__ ExpfExceptions exc_table() { return ExpfExceptions{kE
 
 // Relative error band of the hardware estimate p = v_exp_f32(x * log2(e)) of expf(x): rounding
 // x * log2(e) costs |x| 2^-23 and v_exp_f32 2^-23, together below 2^-16.3 for |x| <= 105 (expf
-// underflows to 0 below -104).  The band 2^-14 leaves a factor 4.9 of margin.
+// underflows to 0 below -104).  The band 2^-14 leaves a factor 4.9 of margin.  The bound holds for
+// normal results only; a denormal or flushed estimate (P = p * 2^24 < 2^-102) can only matter for
+// k = 0, and the accept threshold fma(P, 1 - band, -2^-100) is <= 0 there, so such a case never
+// takes the fast accept: it is rejected (k >= 1 > P) or goes to the exact path.
 constexpr float kBand = 0x1p-14f;
+__device__ __forceinline__ float accept_below(float P) { return __fmaf_rn(P, 1.0f - kBand, -0x1p-100f); }
+__device__ __forceinline__ float reject_above(float P) { return P * (1.0f + kBand); }
 
 // u < expf(x) for u = (w >> 8) * 2^-24, decided with the hardware exp2 unless u falls within the
 // band around the estimate; inside the band the exact double-precision path decides.
@@ -63,8 +68,8 @@ __device__ __forceinline__ bool accept_u(float x, uint32_t w) {
   const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   const float P = p * 16777216.0f;
   const float kf = (float)k;
-  if (kf < P * (1.0f - kBand)) return true;
-  if (kf > P * (1.0f + kBand)) return false;
+  if (kf < accept_below(P)) return true;
+  if (kf > reject_above(P)) return false;
   return accept_exact(x, k, exc_table());
 }
 
@@ -744,7 +749,11 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       const bool valid = r < c.R;
       const float beta = valid ? c.beta_of[r] : 0.0f;
       const uint32_t rho = X.chain0 * c.B + r;
-      const U4 u = philox4x32_10(t, grp, rho, kTagSweep, c.key0, c.key1);
+      // re-materialise the key each task: otherwise the ten round keys are hoisted out of the
+      // loop as twenty live scalars, which the register allocator spills to VGPR lanes
+      uint32_t k0 = c.key0, k1 = c.key1;
+      asm volatile("" : "+s"(k0), "+s"(k1));
+      const U4 u = philox4x32_10(t, grp, rho, kTagSweep, k0, k1);
       uint32_t site[4];
       // the four sites of a group are mutually uncoupled: gather all fields first (independent
       // LDS reads in flight together), then decide and write the new spin words
@@ -769,7 +778,9 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
             f = __fmaf_rn(__uint_as_float(rec[8u + 4u * D + s * D + k]),
                           __uint_as_float(0x3f800000u | ((w[k] << up) & 0x80000000u)), f);
           d[s] = f;
-          bi[s] = (site[s] != kNone) ? (c.sp[(site[s] * G + g) * 2u + half] >> sh) & 1u : 0u;
+          // padded sites read site 0's word: their decision is discarded
+          const uint32_t si = (site[s] != kNone) ? site[s] : 0u;
+          bi[s] = (c.sp[(si * G + g) * 2u + half] >> sh) & 1u;
         }
       } else {
 #pragma unroll
@@ -795,7 +806,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         ks[s] = pick(u, s) >> 8;
         const float P = __builtin_amdgcn_exp2f(xs[s] * 1.44269504088896341f) * 16777216.0f;
         const float kf = (float)ks[s];
-        const bool neg = dd[s] < 0.0f, lo = kf < P * (1.0f - kBand), hi = kf > P * (1.0f + kBand);
+        const bool neg = dd[s] < 0.0f, lo = kf < accept_below(P), hi = kf > reject_above(P);
         ok[s] = neg || lo;
         band[s] = !neg && !lo && !hi && valid && site[s] != kNone;
         any_band |= band[s];
@@ -812,7 +823,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         const uint32_t i = site[s];
         const bool acc_ok = ok[s] && valid;
         const uint64_t nw = ballot((bi[s] ^ (acc_ok ? 1u : 0u)) != 0u);
-        if (sh == 0u) c.sp[(i * G + g) * 2u + half] = half ? (uint32_t)(nw >> 32) : (uint32_t)nw;
+        if (c.lane == 0u) *reinterpret_cast<uint64_t *>(c.sp + (i * G + g) * 2u) = nw;
         if (acc_ok) a_g += dd[s];
         if (self) {
           const float sj = c4(X.self_J)[i];
@@ -821,9 +832,13 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       }
       // the group's accepted dE, summed in group order, enter the fixed-point total
       const long long fx = to_fix(a_g);
+      if (G == 1u) {
+        acc[0] += fx;
+      } else {
 #pragma unroll
-      for (uint32_t gg = 0; gg < 4u; ++gg)
-        if (gg == g) acc[gg] += fx;
+        for (uint32_t gg = 0; gg < 4u; ++gg)
+          if (gg == g) acc[gg] += fx;
+      }
     }
     a = b;
     __syncthreads();

@@ -42,7 +42,8 @@ __device__ __forceinline__ bool sa_accept(float x, uint32_t w) {
   const float p = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
   const float P = p * 16777216.0f;
   const float kf = (float)k;
-  if (kf < P * (1.0f - 0x1p-12f)) return true;
+  // the -2^-100 keeps a denormal or flushed estimate (possible only for k = 0) off the fast accept
+  if (kf < __fmaf_rn(P, 1.0f - 0x1p-12f, -0x1p-100f)) return true;
   if (kf > P * (1.0f + 0x1p-12f)) return false;
   return accept_exact(x, k, ExpfExceptions{kExpfExcX, kExpfExcY, kExpfExcCount});
 }
