@@ -797,24 +797,28 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       }
       float dd[4], xs[4];
       uint32_t ks[4];
-      bool ok[4], band[4];
-      bool any_band = false;
+      bool ok[4];
+      uint64_t band[4], any_band = 0ull;
 #pragma unroll
       for (uint32_t s = 0; s < 4u; ++s) {
         dd[s] = d[s] * (bi[s] ? 2.0f : -2.0f);  // dE *= -2*s_i
         xs[s] = -beta * dd[s];
         ks[s] = pick(u, s) >> 8;
-        const float P = __builtin_amdgcn_exp2f(xs[s] * 1.44269504088896341f) * 16777216.0f;
+        // p = 2^-24 P; the thresholds scale by the exact constants 2^24 (1 -+ band), so the
+        // products round exactly as accept_below(P) and reject_above(P) do
+        const float p = __builtin_amdgcn_exp2f(xs[s] * 1.44269504088896341f);
         const float kf = (float)ks[s];
-        const bool neg = dd[s] < 0.0f, lo = kf < accept_below(P), hi = kf > reject_above(P);
+        const bool neg = dd[s] < 0.0f;
+        const bool lo = kf < __fmaf_rn(p, 16777216.0f * (1.0f - kBand), -0x1p-100f);
+        const bool hi = kf > p * (16777216.0f * (1.0f + kBand));
         ok[s] = neg || lo;
-        band[s] = !neg && !lo && !hi && valid && site[s] != kNone;
+        band[s] = ballot(!neg && !lo && !hi && valid && site[s] != kNone);
         any_band |= band[s];
       }
-      if (ballot(any_band) != 0ull) {
+      if (any_band != 0ull) {
 #pragma unroll
         for (uint32_t s = 0; s < 4u; ++s)
-          if (band[s]) ok[s] = accept_exact(xs[s], ks[s], exc_table());
+          if ((band[s] >> c.lane) & 1ull) ok[s] = accept_exact(xs[s], ks[s], exc_table());
       }
       float a_g = 0.f;
 #pragma unroll
