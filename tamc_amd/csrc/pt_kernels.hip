@@ -157,7 +157,7 @@ struct Ctx {
   uint32_t *sp;      // LDS bit planes
   float *energy, *beta_of, *dEx;
   long long *partial, *drift;
-  uint32_t *perm, *pair_a, *pair_b, *seed_blk, *seed_rank, *misc;
+  uint32_t *perm, *pair_a, *pair_b, *seed_blk, *misc;
   uint32_t *moves;   // LDS count of ICM moves in this launch
   uint8_t *tags;
   uint32_t *nacc, *hist, *rt;
@@ -165,8 +165,19 @@ struct Ctx {
   const uint16_t *nbr;  // ICM neighbour lists [n][D] (LDS copy or global)
   bool nbr_in_lds;
   long long *dEa;
+  unsigned long long *ticks;  // LDS phase timers [kPhaseSlots] and the last timestamp
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
+  bool icm_lds;
 };
+
+// Phase timer of wave 0: adds the time since the previous lap to slot ph.
+__device__ __forceinline__ void lap(const Ctx &c, int ph) {
+  if (c.tid == 0u) {
+    const unsigned long long now = wall_clock64();
+    c.ticks[ph] += now - c.ticks[kPhaseSlots];
+    c.ticks[kPhaseSlots] = now;
+  }
+}
 
 // Spin bits of one site for replica indices chosen at run time.  With one 64-bit plane word per
 // site (at most 64 replicas) the word is held in a register and a bit is one 64-bit shift; wider
@@ -391,7 +402,7 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
 // 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
 // the two chains; bit p of the per-site words qw/cw belongs to pair p.
 template <bool kLds>
-__device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long *ticks, unsigned long long &tk) {
+__device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
   using U32 = typename IcmMem<kLds>::U32;
   IcmMem<kLds> M;
@@ -408,45 +419,50 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     c.dEa[c.tid] = 0;
   }
   __syncthreads();
-  // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; per-block counts
+  // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; blk[p][b] =
+  //     number of such sites in 64-site block b
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
     const uint32_t i = blk * 64u + c.lane;
     const bool valid = i < n;
-    SiteWords h{0ull, c.sp, 0u, G};
-    if (valid) h.load(c.sp, i, G);
     uint32_t q = 0;
-    for (uint32_t p0 = 0; p0 < P; p0 += 4u) {
-      const uint4 A = reinterpret_cast<const uint4 *>(c.pair_a)[p0 >> 2];
-      const uint4 Bp = reinterpret_cast<const uint4 *>(c.pair_b)[p0 >> 2];
-      const uint32_t av[4] = {A.x, A.y, A.z, A.w}, bv[4] = {Bp.x, Bp.y, Bp.z, Bp.w};
-#pragma unroll
-      for (uint32_t s = 0; s < 4u; ++s) {
-        const uint32_t p = p0 + s;
-        if (p >= P) break;
-        const uint32_t qb = valid ? (h.bit(av[s]) ^ h.bit(bv[s])) : 0u;
-        q |= qb << p;
-        const uint32_t cnt = popc64(ballot(qb != 0u));
-        if (c.lane == 0) M.blk[blk * P + p] = cnt;
-      }
-    }
     if (valid) {
+      SiteWords h{0ull, c.sp, 0u, G};
+      h.load(c.sp, i, G);
+      for (uint32_t p0 = 0; p0 < P; p0 += 4u) {
+        const uint4 A = reinterpret_cast<const uint4 *>(c.pair_a)[p0 >> 2];
+        const uint4 Bp = reinterpret_cast<const uint4 *>(c.pair_b)[p0 >> 2];
+        const uint32_t av[4] = {A.x, A.y, A.z, A.w}, bv[4] = {Bp.x, Bp.y, Bp.z, Bp.w};
+#pragma unroll
+        for (uint32_t s = 0; s < 4u; ++s) {
+          const uint32_t p = p0 + s;
+          if (p >= P) break;
+          q |= (h.bit(av[s]) ^ h.bit(bv[s])) << p;
+        }
+      }
       M.qw[i] = q;
       M.cw[i] = 0u;
     }
+    // lane p collects pair p's count of the block
+    uint32_t cnt = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint32_t cp = popc64(ballot((q >> p) & 1u));
+      cnt = (c.lane == p) ? cp : cnt;
+    }
+    if (c.lane < P) M.blk[c.lane * c.nblk + blk] = cnt;
   }
   __syncthreads();
-  {
-    const unsigned long long now = wall_clock64();
-    ticks[kPhIcmOverlap] += now - tk;
-    tk = now;
-  }
-  // (b) seed choice: k = floor(w * count / 2^32) into the ascending list of q<0 sites
-  if (c.wave == 0 && c.lane < P) {
-    const uint32_t p = c.lane;
+  lap(c, kPhIcmOverlap);
+  // (b) seed choice, one wave per pair: k = floor(w * count / 2^32) into the ascending list of the
+  //     pair's q<0 sites.  Lane j holds block j's count (64 blocks at a time); a wave scan finds the
+  //     block holding the k-th site and a ballot over that block the site, which joins cw.
+  for (uint32_t p = c.wave; p < P; p += c.W) {
     uint32_t total = 0;
-#pragma unroll 8
-    for (uint32_t blk = 0; blk < c.nblk; ++blk) total += M.blk[blk * P + p];
-    uint32_t sb = kNone, sr = 0;
+    for (uint32_t c0 = 0; c0 < c.nblk; c0 += 64u) {
+      const uint32_t j = c0 + c.lane;
+      const uint32_t v = (j < c.nblk) ? M.blk[p * c.nblk + j] : 0u;
+      total += (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
+    }
+    uint32_t sb = kNone;
     if (total) {
       uint32_t k;
       if (X.icm_k) {
@@ -457,39 +473,35 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
         k = (uint32_t)(((uint64_t)v.x * total) >> 32);
       }
       uint32_t cum = 0;
-      for (uint32_t blk = 0; blk < c.nblk; ++blk) {
-        const uint32_t cb = M.blk[blk * P + p];
-        if (k < cum + cb) {
-          sb = blk;
-          sr = k - cum;
+      for (uint32_t c0 = 0; c0 < c.nblk; c0 += 64u) {
+        const uint32_t j = c0 + c.lane;
+        const uint32_t v = (j < c.nblk) ? M.blk[p * c.nblk + j] : 0u;
+        const uint32_t incl = wave_incl_scan(v);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        if (k < cum + tot) {
+          const uint32_t r = k - cum;
+          const uint32_t jj = (uint32_t)__builtin_ctzll(ballot(incl > r));
+          const uint32_t rank = r - ((uint32_t)__builtin_amdgcn_readlane((int)incl, (int)jj) -
+                                     (uint32_t)__builtin_amdgcn_readlane((int)v, (int)jj));
+          sb = c0 + jj;
+          const uint32_t i = sb * 64u + c.lane;
+          const bool qb = i < n && ((M.qw[i] >> p) & 1u);
+          const uint64_t m = ballot(qb);
+          if (qb && mbcnt64(m) == rank) atomicOr((uint32_t *)&M.cw[i], 1u << p);
           break;
         }
-        cum += cb;
+        cum += tot;
       }
+      if (c.lane == 0) atomicAdd(c.moves, 1u);
     }
-    c.seed_blk[p] = sb;
-    c.seed_rank[p] = sr;
+    if (c.lane == 0) c.seed_blk[p] = sb;
   }
-  __syncthreads();
-  for (uint32_t p = c.wave; p < P; p += c.W) {
-    const uint32_t sb = c.seed_blk[p];
-    if (sb == kNone) continue;
-    const uint32_t i = sb * 64u + c.lane;
-    const uint32_t qb = (i < n) ? ((M.qw[i] >> p) & 1u) : 0u;
-    const uint64_t m = ballot(qb != 0u);
-    if (qb && mbcnt64(m) == c.seed_rank[p]) atomicOr((uint32_t *)&M.cw[i], 1u << p);
-    if (c.lane == 0) atomicAdd(c.moves, 1u);
-  }
-  {
-    const unsigned long long now = wall_clock64();
-    ticks[kPhIcmSeed] += now - tk;
-    tk = now;
-  }
+  lap(c, kPhIcmSeed);
   // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
   //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing.
   for (;;) {
     __syncthreads();
-    ticks[kPhIcmPasses] += 1u;
+    if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
     int changed = 0;
     // two rounds over the sites between barriers: updates made by other waves during the first
     // round are already visible in the second, so clusters grow by more than one hop per barrier
@@ -527,11 +539,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t, unsigned long long
     }
     if (!__syncthreads_or(changed)) break;
   }
-  {
-    const unsigned long long now = wall_clock64();
-    ticks[kPhIcmGrow] += now - tk;
-    tk = now;
-  }
+  lap(c, kPhIcmGrow);
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
   switch (X.ell_D) {
     case 3u: icm_flip<3>(c, M); break;
@@ -985,16 +993,14 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
 // One work group per PtGroup.  Sweeps [t0, t1) of every group, or, when `ranges` is given,
 // sweeps [ranges[2b], ranges[2b+1]) of group b (time slices of a serving loop).  When
 // `launch_ticks` is given, thread 0 stores the group's 100 MHz tick count for this launch there.
-__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
-                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
-                                                        unsigned long long *launch_ticks) {
+// Kernel context of the calling work group: sizes, key, LDS layout pointers.  Cheap to build (a
+// few scalar loads and integer ops), so the sweep loop rebuilds it every sweep from the group
+// pointer passed through an empty asm statement: that keeps the compiler from holding dozens of
+// loop-invariant scalars live across all phases, which it would otherwise spill to VGPR lanes and
+// read back with v_readlane inside the hot loops.
+__device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds, uint32_t lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   const TAMC_AS4 PtGroup &X = *c4(groups + blockIdx.x);
-  if (ranges) {
-    t0 = c4(ranges)[2u * blockIdx.x];
-    t1 = c4(ranges)[2u * blockIdx.x + 1u];
-  }
-  const unsigned long long launch_start = wall_clock64();
   Ctx c;
   c.X = &X;
   c.n = X.n;
@@ -1014,6 +1020,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.key0 = (uint32_t)X.seed;
   c.key1 = (uint32_t)(X.seed >> 32);
   const bool use_lds_icm = icm_lds != 0u && !(X.flags & kFlagIcmGlobal);
+  c.icm_lds = use_lds_icm;
   const LdsLayout L = with_nbr(make_layout(c.n, c.G, c.R, c.B, c.P, c.W, use_lds_icm), c.n,
                                X.ell_col16 ? X.ell_D : 0u, lds_bytes);
   c.sp = (uint32_t *)(smem + L.spins);
@@ -1027,7 +1034,6 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.pair_a = (uint32_t *)(smem + L.pair_a);
   c.pair_b = (uint32_t *)(smem + L.pair_b);
   c.seed_blk = (uint32_t *)(smem + L.seed_blk);
-  c.seed_rank = (uint32_t *)(smem + L.seed_rank);
   c.misc = (uint32_t *)(smem + L.misc);
   c.partial = (long long *)(smem + L.partial);
   c.dEx = (float *)(smem + L.dE);
@@ -1047,16 +1053,35 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   c.nbr_in_lds = false;
   if (c.P > 0 && X.ell_col16 != nullptr && X.ell_D != 0u) {
     if (L.nbr != kNone) {
-      uint16_t *nb = (uint16_t *)(smem + L.nbr);
-      for (uint32_t k = c.tid; k < c.n * nbr_stride(X.ell_D); k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
-      c.nbr = nb;
+      c.nbr = (const uint16_t *)(smem + L.nbr);
       c.nbr_in_lds = true;
     } else {
       c.nbr = X.ell_col16;
     }
   }
   c.moves = c.misc + 4;
-  if (c.tid == 0) c.moves[0] = 0u;
+  c.ticks = (unsigned long long *)(smem + L.ticks);
+  return c;
+}
+
+__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
+                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
+                                                        unsigned long long *launch_ticks) {
+  if (ranges) {
+    t0 = c4(ranges)[2u * blockIdx.x];
+    t1 = c4(ranges)[2u * blockIdx.x + 1u];
+  }
+  const unsigned long long launch_start = wall_clock64();
+  Ctx c = make_ctx(groups, icm_lds, lds_bytes);
+  const TAMC_AS4 PtGroup &X = *c.X;
+  if (c.P > 0 && c.nbr_in_lds) {
+    uint16_t *nb = const_cast<uint16_t *>(c.nbr);
+    for (uint32_t k = c.tid; k < c.n * nbr_stride(X.ell_D); k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
+  }
+  if (c.tid == 0) {
+    c.moves[0] = 0u;
+    for (int k = 0; k <= (int)kPhaseSlots; ++k) c.ticks[k] = 0ull;
+  }
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
   for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
@@ -1076,29 +1101,28 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   for (uint32_t k = c.tid; k < c.nch * c.B; k += c.nthreads) c.beta_of[c.perm[k]] = c4(X.betas)[k % c.B];
   __syncthreads();
   const uint32_t tend = t1 < X.T ? t1 : X.T;
-  unsigned long long ticks[kPhaseSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  unsigned long long tk = wall_clock64();
-  auto lap = [&](int ph) {
-    const unsigned long long now = wall_clock64();
-    ticks[ph] += now - tk;
-    tk = now;
-  };
+  if (c.tid == 0) c.ticks[kPhaseSlots] = wall_clock64();
   for (uint32_t t = t0; t < tend; ++t) {
-    if ((c.flags & kFlagIcm) && c.P > 0) {
-      if (use_lds_icm) icm_phase<true>(c, t, ticks, tk);
-      else icm_phase<false>(c, t, ticks, tk);
+    const PtGroup *gp = groups;
+    uint32_t il = icm_lds, lb = lds_bytes;
+    asm volatile("" : "+s"(gp), "+s"(il), "+s"(lb));
+    Ctx cs = make_ctx(gp, il, lb);
+    if ((cs.flags & kFlagIcm) && cs.P > 0) {
+      if (cs.icm_lds) icm_phase<true>(cs, t);
+      else icm_phase<false>(cs, t);
     }
-    lap(kPhIcmFlip);
-    if (c.flags & kFlagIcmOnly) break;
-    exchange_phase(c, t);
-    lap(kPhExchange);
-    sweep_phase(c, t);
-    lap(kPhSweep);
-    measure_phase(c, t);
-    lap(kPhMeasure);
+    lap(cs, kPhIcmFlip);
+    if (cs.flags & kFlagIcmOnly) break;
+    exchange_phase(cs, t);
+    lap(cs, kPhExchange);
+    sweep_phase(cs, t);
+    lap(cs, kPhSweep);
+    measure_phase(cs, t);
+    lap(cs, kPhMeasure);
   }
+  __syncthreads();
   if (c.tid == 0 && X.phase_ticks)
-    for (int k = 0; k < (int)kPhaseSlots; ++k) X.phase_ticks[k] += ticks[k];
+    for (int k = 0; k < (int)kPhaseSlots; ++k) X.phase_ticks[k] += c.ticks[k];
   __syncthreads();
   if (c.tid == 0) X.icm_moves[0] += c.moves[0];
   if (c.tid == 0 && launch_ticks) launch_ticks[blockIdx.x] = wall_clock64() - launch_start;

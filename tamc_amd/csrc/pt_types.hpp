@@ -95,7 +95,7 @@ struct PtGroup {
 };
 
 struct LdsLayout {
-  uint32_t spins, energy, beta_of, perm, tags, nacc, hist, rt, pair_a, pair_b, pair_of, seed_blk, seed_rank, misc, partial, dE, drift,
+  uint32_t spins, energy, beta_of, perm, tags, nacc, hist, rt, pair_a, pair_b, pair_of, seed_blk, misc, ticks, partial, dE, drift,
       uw, dEa, qw, cw, blk, nbr, total;
 };
 
@@ -131,10 +131,10 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(R * 4u);
   L.seed_blk = o;
   o += lds_align(kMaxPairs * 4u);
-  L.seed_rank = o;
-  o += lds_align(kMaxPairs * 4u);
   L.misc = o;
   o += lds_align(16u * 4u);
+  L.ticks = o;  // phase timers [kPhaseSlots] + last timestamp
+  o += lds_align((kPhaseSlots + 1u) * 8u);
   L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G)
   o += lds_align(W * G * 64u * 8u);
   L.dE = o;  // exchange energy differences [2][B]
@@ -150,7 +150,7 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
     o += lds_align(n * 4u);
     L.cw = o;
     o += lds_align(n * 4u);
-    L.blk = o;
+    L.blk = o;  // per-pair counts of q<0 sites in each 64-site block, [P][nblk]
     o += lds_align(nblk * P * 4u);
   } else {
     L.qw = L.cw = L.blk = kNone;
