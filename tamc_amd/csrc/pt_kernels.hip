@@ -425,7 +425,13 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     const uint32_t i = blk * 64u + c.lane;
     const bool valid = i < n;
     uint32_t q = 0;
-    if (valid) {
+    if (valid && G == 1u) {
+      // one 64-bit plane word per site: bit p = bit a_p ^ bit b_p of the word
+      const uint64_t w = *reinterpret_cast<const uint64_t *>(c.sp + 2u * i);
+      for (uint32_t p = 0; p < P; ++p) q |= (uint32_t)(((w >> c.pair_a[p]) ^ (w >> c.pair_b[p])) & 1ull) << p;
+      M.qw[i] = q;
+      M.cw[i] = 0u;
+    } else if (valid) {
       SiteWords h{0ull, c.sp, 0u, G};
       h.load(c.sp, i, G);
       for (uint32_t p0 = 0; p0 < P; p0 += 4u) {
@@ -442,11 +448,15 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
       M.qw[i] = q;
       M.cw[i] = 0u;
     }
-    // lane p collects pair p's count of the block
+    // per-pair counts of the block, four pairs at a time: bits 4k..4k+3 of q spread into the low
+    // bits of four bytes (the multiply places copies of the nibble at bits 0, 7, 14 and 21, which
+    // cannot carry into each other), summed over the wave (each byte <= 64); lane p keeps byte
+    // p % 4 of word p / 4
     uint32_t cnt = 0;
-    for (uint32_t p = 0; p < P; ++p) {
-      const uint32_t cp = popc64(ballot((q >> p) & 1u));
-      cnt = (c.lane == p) ? cp : cnt;
+    for (uint32_t k = 0; 4u * k < P; ++k) {
+      const uint32_t v = __umul24((q >> (4u * k)) & 0xFu, 0x00204081u) & 0x01010101u;
+      const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), 63);
+      if ((c.lane >> 2) == k) cnt = (tot >> (8u * (c.lane & 3u))) & 0xFFu;
     }
     if (c.lane < P) M.blk[c.lane * c.nblk + blk] = cnt;
   }
