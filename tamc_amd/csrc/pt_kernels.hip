@@ -166,6 +166,8 @@ struct Ctx {
   bool nbr_in_lds;
   long long *dEa;
   unsigned long long *ticks;  // LDS phase timers [kPhaseSlots] and the last timestamp
+  unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [k][v] = replicas of
+                              // the pairs 4k + j with bit j of v set (one plane word per site)
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
   bool icm_lds;
 };
@@ -349,6 +351,15 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   // lane = site: flip replicas a_p, b_p for every pair p whose cluster holds the site
   auto flip_block = [&](uint32_t blk, uint32_t cwi) {
     const uint32_t i_lane = blk * 64u + c.lane;
+    if (G == 1u) {
+      unsigned long long mg = 0ull;
+      for (uint32_t k = 0; 4u * k < P; ++k) mg |= c.fmask[16u * k + ((cwi >> (4u * k)) & 15u)];
+      if (mg) {
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(c.sp) + i_lane;
+        *w ^= mg;
+      }
+      return;
+    }
     for (uint32_t g = 0; g < G; ++g) {
       uint64_t mg = 0ull;
       for (uint32_t p = 0; p < P; ++p) {
@@ -419,6 +430,15 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     c.dEa[c.tid] = 0;
   }
   __syncthreads();
+  if (G == 1u && c.tid < 16u * ((P + 3u) / 4u)) {
+    const uint32_t k = c.tid >> 4, v = c.tid & 15u;
+    unsigned long long m = 0ull;
+    for (uint32_t j = 0; j < 4u; ++j) {
+      const uint32_t p = 4u * k + j;
+      if (p < P && ((v >> j) & 1u)) m |= (1ull << c.pair_a[p]) | (1ull << c.pair_b[p]);
+    }
+    c.fmask[c.tid] = m;  // read only after the growth loop's barriers
+  }
   // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; blk[p][b] =
   //     number of such sites in 64-site block b
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
@@ -1071,6 +1091,7 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds,
   }
   c.moves = c.misc + 4;
   c.ticks = (unsigned long long *)(smem + L.ticks);
+  c.fmask = (unsigned long long *)(smem + L.fmask);
   return c;
 }
 
