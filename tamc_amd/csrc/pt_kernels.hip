@@ -166,6 +166,7 @@ struct Ctx {
   bool nbr_in_lds;
   long long *dEa;
   unsigned long long *ticks;  // LDS phase timers [kPhaseSlots] and the last timestamp
+  uint32_t *fscr;             // per-wave 64-word scratch of the ICM flip
   unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [k][v] = replicas of
                               // the pairs 4k + j with bit j of v set (one plane word per site)
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
@@ -213,6 +214,17 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 }
 
 // value of v in lane `src` (0..63); every lane of the wave must be active
+// Inclusive running maximum over the 64 lanes (same DPP pattern as wave_incl_scan; 0 is neutral).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return v;
+}
+
 __device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t src) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
@@ -287,17 +299,23 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   };
   auto round = [&](uint32_t blk, uint32_t cwi, uint32_t excl, uint32_t total, uint32_t base, const SiteData &sd) {
     const uint32_t m = base + c.lane;
-    // the item's site: the last lane whose range starts at or before m
-    uint32_t s = 0, start = 0;
-#pragma unroll
-    for (uint32_t step = 32u; step >= 1u; step >>= 1) {
-      const uint32_t probe = s + step;
-      const uint32_t v = lane_value(excl, probe & 63u);
-      if (probe < 64u && v <= m) {
-        s = probe;
-        start = v;
-      }
-    }
+    // the item's site is the last lane s with excl_s <= m.  Every site whose range starts inside
+    // the round posts (excl_s, s) at its start position; a running maximum carries it over the
+    // site's items.  The site covering item `base` (the last with excl_s <= base, non-empty since
+    // base < total) seeds position 0.
+    uint32_t *buf = c.fscr + 64u * c.wave;
+    buf[c.lane] = 0u;
+    if (cwi != 0u && excl >= base && excl < base + 64u) buf[excl - base] = (excl << 8) | c.lane;
+    // lanes read what other lanes wrote: fence at wavefront scope so that the compiler neither
+    // forwards a lane's own zero nor moves the read
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t s0 = 63u - (uint32_t)__builtin_clzll(ballot(excl <= base));
+    uint32_t v = buf[c.lane];
+    if (c.lane == 0u) v = max(v, ((uint32_t)__builtin_amdgcn_readlane((int)excl, (int)s0) << 8) | s0);
+    v = wave_incl_max(v);
+    const uint32_t s = v & 0xFFu, start = v >> 8;
     const uint32_t cws = lane_value(cwi, s);
     const float bias_s = __uint_as_float(lane_value(sd.bias, s));
     uint32_t nlo = 0u, nhi = 0u;
@@ -1092,6 +1110,7 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds,
   c.moves = c.misc + 4;
   c.ticks = (unsigned long long *)(smem + L.ticks);
   c.fmask = (unsigned long long *)(smem + L.fmask);
+  c.fscr = (uint32_t *)(smem + L.fscr);
   return c;
 }
 
