@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# aggregate LDS bandwidth assumed for the LDS-level fraction: 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
 BYTES_PER_ATTEMPT_D3 = 9 * 3 + 9  # 4 B col + 4 B J + 1 B spin per edge; 4 B row_ptr + 4 B bias + 1 B spin per site
 
 
@@ -290,7 +292,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic() if default_workload else None,
                      "bytes_per_attempt": BYTES_PER_ATTEMPT_D3,
                      "algorithmic_bytes_per_launch": attempts_per_step * BYTES_PER_ATTEMPT_D3,
-                     "kernel_ms": float(np.mean(dev_ms))},
+                     "kernel_ms": float(np.mean(dev_ms)),
+                     # the state lives in LDS for the whole launch: the same bytes against LDS
+                     "lds_level": {"peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": achieved / LDS_PEAK_GBS}},
     }
     if not args.no_other:
         out["other_configs"] = other_configs(not args.no_cpu)

@@ -1,8 +1,6 @@
 """The `tamc` command line end to end on the GPU: the files it writes hold the same results as the
 runner API on the same seed (TAMC_SEED), in the reference's formats."""
 import os
-import pickle
-import struct
 import subprocess
 import tempfile
 
@@ -10,7 +8,7 @@ import numpy as np
 import pytest
 import yaml
 
-from tamc_amd import api, workloads as W
+from tamc_amd import api, samples, workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -33,26 +31,6 @@ PT:
   sample_states: 16
   sample_limiting: 1
 """
-
-
-def read_thermal_bincode(path):
-    data = open(path, "rb").read()
-    off = 0
-
-    def take(fmt):
-        nonlocal off
-        v = struct.unpack_from("<" + fmt, data, off)
-        off += struct.calcsize("<" + fmt)
-        return v
-
-    out = {"compression_level": take("B")[0], "instance_size": take("Q")[0]}
-    out["beta_arr"] = list(take("%df" % take("Q")[0]))
-    out["samples"] = [[list(take("%dB" % take("Q")[0])) for _ in range(take("Q")[0])] for _ in range(take("Q")[0])]
-    out["e"] = [list(take("%df" % take("Q")[0])) for _ in range(take("Q")[0])]
-    out["q"] = [list(take("%di" % take("Q")[0])) for _ in range(take("Q")[0])]
-    out["suscept"] = [[list(take("%df" % take("Q")[0])) for _ in range(take("Q")[0])] for _ in range(take("Q")[0])]
-    assert off == len(data)
-    return out
 
 
 def cli(args, d, seed):
@@ -90,12 +68,11 @@ def test_cli_pt_outputs_match_runner(gpu):
     assert y["acceptance_counts"] == [int(x) for x in ref["acceptance_counts"]]
     assert y["instance_size"] == 64 and y["num_measurements"] == 0 and y["timing"] > 0
 
-    b = read_thermal_bincode(os.path.join(d, "samples.bin"))
-    with open(os.path.join(d, "s.pkl"), "rb") as f:
-        p = pickle.load(f)  # written by this build's CLI
+    b = samples.read_thermal_samples(os.path.join(d, "samples.bin"))
+    p = samples.read_thermal_samples(os.path.join(d, "s.pkl"))  # written by this build's CLI
     for obj in (b, p):
         assert obj["instance_size"] == 64
-        assert np.array_equal(np.array(obj["beta_arr"], np.float32), params.resolve()[0])
+        assert np.array_equal(obj["beta_arr"], params.resolve()[0])
         assert np.array_equal(np.array(obj["e"], np.float32), ref["e"])
         assert np.array_equal(np.array(obj["q"], np.int32), ref["q"])
         assert np.array_equal(np.array(obj["samples"], np.uint8), ref["samples"])
@@ -120,7 +97,9 @@ def test_cli_sa_outputs_match_runner(gpu):
     assert np.float32(y["min_energy"]) == ref["min_energy"]
     assert np.array_equal(np.array(y["energies"], np.float32), ref["energies"])
     assert y["params"]["num_replicas"] == 96 and y["params"]["threads"] == 8
-    data = open(os.path.join(d, "samples.bin"), "rb").read()
-    size, count, nbytes = struct.unpack_from("<IIQ", data, 0)
+    a = samples.read_anneal_state(os.path.join(d, "samples.bin"))
     ngs = int(np.sum(ref["energies"] <= ref["min_energy"] + np.float32(1e-8)))
-    assert size == 40 and count == ngs and nbytes == ngs * 5 and len(data) == 16 + nbytes
+    assert a["instance_size"] == 40 and a["num_replicas"] == ngs and a["bytes"].size == ngs * 5
+    # the packed ground states decode to states whose energy_ref is the minimum energy
+    states = samples.unpack_states(a["bytes"], 40)
+    assert np.array_equal(inst.energy(states), np.full(ngs, ref["min_energy"], np.float32))
