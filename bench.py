@@ -239,11 +239,13 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    dev_ms = []
+    dev_ms, prep_ms, post_ms = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res, st = step()
         dev_ms.append(st["device_ms"])
+        prep_ms.append(st["prep_ms"])
+        post_ms.append(st["post_ms"])
         assert st["attempts"] == attempts_per_step
     wall = time.perf_counter() - t0
     value = attempts_per_step * args.steps / wall
@@ -285,6 +287,11 @@ def main():
             "launches": st["launches"],
         },
         "kernel_attempts_per_s": kernel_rate,
+        # where a step's time outside the kernel goes: library preparation (planning, allocation,
+        # uploads, state init) and post-processing (downloads, assembly); the rest is Python
+        "host_ms_per_step": {"prep": float(np.mean(prep_ms)), "post": float(np.mean(post_ms)),
+                             "other": ms_per_step - float(np.mean(dev_ms)) - float(np.mean(prep_ms))
+                             - float(np.mean(post_ms))},
         "plain_gpu": {"value": plain_rate, "unit": "spin-flip attempts/s", "instances": k,
                       "note": "one instance per kernel launch, launched one after another"},
         "batched_vs_plain": (value / plain_rate) if plain_rate else None,

@@ -180,6 +180,9 @@ typedef struct {
   double phase_ms[7];        /* slowest work group: ICM overlap words, ICM seeds, ICM cluster growth,
                                 ICM flip, exchange, sweep, measurement time */
   double icm_passes;         /* slowest work group: ICM cluster-growth passes, summed over sweeps */
+  double prep_ms;            /* host time from the call to the first sweep launch (planning,
+                                allocation, uploads, state initialisation) */
+  double post_ms;            /* host time after the last sweep (downloads, result assembly) */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

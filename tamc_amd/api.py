@@ -183,9 +183,12 @@ class PtIcmParams:
 
 
 def _arr(p, count, dtype):
+    # one copy of count elements out of library memory (string_at + frombuffer is several times
+    # cheaper per array than numpy.ctypeslib.as_array + copy, which matters for large batches)
+    dt = np.dtype(dtype)
     if count == 0:
-        return np.zeros(0, dtype)
-    return np.ctypeslib.as_array(p, shape=(count,)).copy().astype(dtype)
+        return np.zeros(0, dt)
+    return np.frombuffer(bytearray(C.string_at(p, count * dt.itemsize)), dt)
 
 
 def pt_result_to_dict(R):
@@ -285,7 +288,7 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
     st = dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
-              icm_passes=stats.icm_passes)
+              icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms)
     return res, st
 
 

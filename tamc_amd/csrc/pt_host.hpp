@@ -51,11 +51,12 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
                        uint32_t waves);
 // Dynamic LDS bytes the sweep kernel needs for group g at `waves` waves.
 uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves);
-// Register every device buffer of S in the arena (the ground-state trace in Ags when given).
-void want_buffers(SolvePlan &S, Arena &A, Arena *Ags = nullptr);
-// After the arena is placed: upload the betas (on stream s, or synchronously when s is null) and
-// fill the groups' PtGroup descriptors.
-tamc_status fill_groups(SolvePlan &S, hipStream_t s);
+// Register every device buffer of S in the arena (the ground-state trace in Ags and the betas in
+// Ab when given).
+void want_buffers(SolvePlan &S, Arena &A, Arena *Ags = nullptr, Arena *Ab = nullptr);
+// After the arena is placed: upload the betas unless the caller does (on stream s, or
+// synchronously when s is null) and fill the groups' PtGroup descriptors.
+tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas = true);
 // Host copy of a device range: result arrays inside it are read from `host`.
 struct HostMirror {
   const char *host;

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <functional>
@@ -13,6 +14,7 @@
 #include <mutex>
 #include <tuple>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pt_host.hpp"
@@ -95,8 +97,8 @@ uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
       .total;
 }
 
-void want_buffers(SolvePlan &S, Arena &A, Arena *Ags) {
-  A.want(&S.d_betas, S.B);
+void want_buffers(SolvePlan &S, Arena &A, Arena *Ags, Arena *Ab) {
+  (Ab ? *Ab : A).want(&S.d_betas, S.B);
   for (auto &g : S.groups) {
     PtGroup &X = g.g;
     const uint32_t nb = (S.n + 63) / 64;
@@ -127,9 +129,9 @@ void want_buffers(SolvePlan &S, Arena &A, Arena *Ags) {
   }
 }
 
-tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
-  if (s) TAMC_HIP(hipMemcpyAsync(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice, s));
-  else TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
+tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
+  if (upload_betas && s) TAMC_HIP(hipMemcpyAsync(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice, s));
+  else if (upload_betas) TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
   for (auto &g : S.groups) {
     tamc_status st;
     DeviceInstance *D = device_instance(S.inst, g.G, &st, S.P.sweep_order);
@@ -186,12 +188,17 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s) {
 }
 
 // Allocate device buffers for all groups of all solves and fill their PtGroup descriptors.
-static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A, Arena &Ags) {
-  for (auto &S : solves) want_buffers(S, A, &Ags);
+static tamc_status allocate(std::vector<SolvePlan> &solves, Arena &A, Arena &Ags, Arena &Ab) {
+  for (auto &S : solves) want_buffers(S, A, &Ags, &Ab);
   TAMC_HIP(A.commit());
   TAMC_HIP(Ags.commit());
+  TAMC_HIP(Ab.commit());
+  // every solve's betas in one upload
+  std::vector<char> img(Ab.used, 0);
+  for (auto &S : solves) memcpy(img.data() + ((const char *)S.d_betas - (const char *)Ab.buf.p), S.betas.data(), S.B * 4);
+  if (Ab.used) TAMC_HIP(hipMemcpy(Ab.buf.p, img.data(), Ab.used, hipMemcpyHostToDevice));
   for (auto &S : solves) {
-    const tamc_status st = fill_groups(S, nullptr);
+    const tamc_status st = fill_groups(S, nullptr, false);
     if (st) return st;
   }
   return TAMC_OK;
@@ -346,9 +353,18 @@ tamc_status assemble(const SolvePlan &S, tamc_pt_result **out, const HostMirror 
       const uint32_t c = g.chain0 + cl;
       for (uint32_t b = 0; b < B; ++b) {
         const uint32_t r = perm[cl * B + b];
-        for (uint32_t i = 0; i < n; ++i) {
-          const uint32_t bit = (spins[((size_t)i * g.G + (r >> 6)) * 2 + ((r >> 5) & 1)] >> (r & 31)) & 1u;
-          fin[((size_t)c * B + b) * n + i] = bit ? -1 : 1;
+        int8_t *out_s = &fin[((size_t)c * B + b) * n];
+        if (g.G == 1u) {
+          uint64_t w;
+          for (uint32_t i = 0; i < n; ++i) {
+            memcpy(&w, &spins[(size_t)i * 2], 8);
+            out_s[i] = (int8_t)(1 - 2 * (int)((w >> r) & 1u));
+          }
+        } else {
+          for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t bit = (spins[((size_t)i * g.G + (r >> 6)) * 2 + ((r >> 5) & 1)] >> (r & 31)) & 1u;
+            out_s[i] = bit ? -1 : 1;
+          }
         }
         fe[(size_t)c * B + b] = energy[r];
         ids[(size_t)c * B + b] = g.chain0 * B + r;
@@ -418,13 +434,14 @@ struct RunOptions {
   uint32_t waves = 8;
   uint32_t sweeps_per_launch = 0;  // 0 = all
   bool use_graph = false;
+  std::chrono::steady_clock::time_point t_entry = std::chrono::steady_clock::now();  // for prep_ms
 };
 
 // Plan, allocate, launch and assemble a set of solves that run concurrently in one kernel.
 static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &opt, tamc_pt_result **results,
                               tamc_batch_stats *stats, const std::function<void(std::vector<SolvePlan> &)> &patch = {}) {
-  Arena A, Ags;  // Ags: ground-state trace buffers, large and mostly unused, downloaded by prefix
-  tamc_status st = allocate(solves, A, Ags);
+  Arena A, Ags, Ab;  // Ags: ground-state trace buffers, large and mostly unused, downloaded by prefix
+  tamc_status st = allocate(solves, A, Ags, Ab);
   if (st) return st;
   if (patch) patch(solves);
   Launch Lc;
@@ -473,11 +490,33 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   std::vector<char> mirror_buf(A.used);
   if (A.used) TAMC_HIP(hipMemcpy(mirror_buf.data(), A.buf.p, A.used, hipMemcpyDeviceToHost));
   const HostMirror mirror{mirror_buf.data(), (const char *)A.buf.p, A.used};
-  for (size_t k = 0; k < solves.size(); ++k) {
-    if ((st = assemble(solves[k], &results[k], &mirror))) {
-      for (size_t j = 0; j < k; ++j) tamc_pt_result_free(results[j]);
-      return st;
+  // results are independent: assemble them on several host threads (errors are thread-local, so
+  // each worker keeps its message for this thread to report)
+  const size_t ns = solves.size();
+  std::vector<tamc_status> sts(ns, TAMC_OK);
+  std::vector<std::string> msgs(ns);
+  for (size_t k = 0; k < ns; ++k) results[k] = nullptr;
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t k; (k = next.fetch_add(1)) < ns;) {
+      sts[k] = assemble(solves[k], &results[k], &mirror);
+      if (sts[k]) msgs[k] = tamc_last_error();
     }
+  };
+  const size_t nt = std::min<size_t>(ns, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  for (size_t k = 0; k < ns; ++k)
+    if (sts[k]) {
+      for (size_t j = 0; j < ns; ++j) {
+        tamc_pt_result_free(results[j]);
+        results[j] = nullptr;
+      }
+      return fail(sts[k], msgs[k]);
+    }
+  for (size_t k = 0; k < ns; ++k) {
     results[k]->timing_us = wall_us;
     results[k]->device_ms = ms;
     attempts += results[k]->attempts;
@@ -507,6 +546,8 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
     stats->sweeps_per_launch = spl;
     stats->launches = launches;
     stats->groups = ng;
+    stats->prep_ms = std::chrono::duration<double, std::milli>(w0 - opt.t_entry).count();
+    stats->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w1).count();
   }
   return TAMC_OK;
 }
@@ -648,6 +689,7 @@ static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *in
 tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats) {
   clear_error();
+  const auto t_entry = std::chrono::steady_clock::now();
   if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
   for (uint32_t k = 0; k < count; ++k)
     if (!insts[k]) return fail(TAMC_E_INVALID_ARG, "null instance");
@@ -665,6 +707,7 @@ tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts,
   if (opt.waves < 1 || opt.waves > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
   opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
   opt.use_graph = opts && opts->use_graph;
+  opt.t_entry = t_entry;  // the autotuner's probe launches count as preparation
   std::vector<SolvePlan> solves(count);
   for (uint32_t k = 0; k < count; ++k)
     if ((st = plan_solve(solves[k], insts[k], &params[k], nullptr, opt.waves))) return st;
