@@ -546,46 +546,88 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   }
   lap(c, kPhIcmSeed);
   // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
-  //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing.
-  for (;;) {
-    __syncthreads();
-    if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
-    int changed = 0;
-    // two rounds over the sites between barriers: updates made by other waves during the first
-    // round are already visible in the second, so clusters grow by more than one hop per barrier
-    for (uint32_t rep = 0; rep < 2u; ++rep)
-    for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-      const uint32_t q = M.qw[i];
-      const uint32_t cur = M.cw[i];
-      if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
-      uint32_t nb = 0;
-      if (D != 0u && D <= 4u) {
-        // one 8-byte load of the (padded) row
-        const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
-                                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
-        nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu] | M.cw[w >> 48];
-      } else if (D != 0u) {
-        uint64_t w0, w1;
-        if (M.nbr_lds) {
-          w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i];
-          w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i + 1u];
-        } else {
-          w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i];
-          w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i + 1u];
-        }
-        nb = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu] | M.cw[w0 >> 48] |
-             M.cw[w1 & 0xFFFFu] | M.cw[(w1 >> 16) & 0xFFFFu] | M.cw[(w1 >> 32) & 0xFFFFu] | M.cw[w1 >> 48];
-      } else if (i < X.g_nodes) {
-        const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
-        for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
-      }
-      const uint32_t nc = cur | (q & nb);
-      if (nc != cur) {
-        M.cw[i] = nc;
-        changed = 1;
+  //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing;
+  //     each pass makes two rounds over the sites between barriers: updates made by other waves
+  //     during the first round are already visible in the second, so clusters grow by more than one
+  //     hop per barrier.
+  __syncthreads();
+  constexpr uint32_t kRegSites = 4;
+  if (D != 0u && D <= 4u && n <= kRegSites * c.nthreads) {
+    // every thread owns at most four sites and is the only writer of their cluster words, so it
+    // keeps q, the cluster word and the neighbour row of each in registers: a visit reads only the
+    // neighbours' cluster words (three for rows of width 3, whose fourth entry is the site itself)
+    uint32_t qv[kRegSites], cv[kRegSites];
+    uint64_t rw[kRegSites];
+#pragma unroll
+    for (uint32_t s = 0; s < kRegSites; ++s) {
+      const uint32_t i = c.tid + s * c.nthreads;
+      qv[s] = cv[s] = 0u;
+      rw[s] = 0ull;
+      if (i < n) {
+        qv[s] = M.qw[i];
+        cv[s] = M.cw[i];
+        rw[s] = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                          : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
       }
     }
-    if (!__syncthreads_or(changed)) break;
+    for (;;) {
+      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+      int changed = 0;
+      for (uint32_t rep = 0; rep < 2u; ++rep) {
+#pragma unroll
+        for (uint32_t s = 0; s < kRegSites; ++s) {
+          if (cv[s] == qv[s]) continue;  // nothing left to join (also q == 0 and padding)
+          const uint64_t w = rw[s];
+          uint32_t nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu];
+          if (D == 4u) nb |= M.cw[w >> 48];
+          const uint32_t nc = cv[s] | (qv[s] & nb);
+          if (nc != cv[s]) {
+            cv[s] = nc;
+            M.cw[c.tid + s * c.nthreads] = nc;
+            changed = 1;
+          }
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+  } else {
+    for (;;) {
+      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+      int changed = 0;
+      for (uint32_t rep = 0; rep < 2u; ++rep)
+      for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+        const uint32_t q = M.qw[i];
+        const uint32_t cur = M.cw[i];
+        if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
+        uint32_t nb = 0;
+        if (D != 0u && D <= 4u) {
+          // one 8-byte load of the (padded) row
+          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
+          nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu] | M.cw[w >> 48];
+        } else if (D != 0u) {
+          uint64_t w0, w1;
+          if (M.nbr_lds) {
+            w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i];
+            w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i + 1u];
+          } else {
+            w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i];
+            w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i + 1u];
+          }
+          nb = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu] | M.cw[w0 >> 48] |
+               M.cw[w1 & 0xFFFFu] | M.cw[(w1 >> 16) & 0xFFFFu] | M.cw[(w1 >> 32) & 0xFFFFu] | M.cw[w1 >> 48];
+        } else if (i < X.g_nodes) {
+          const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
+          for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
+        }
+        const uint32_t nc = cur | (q & nb);
+        if (nc != cur) {
+          M.cw[i] = nc;
+          changed = 1;
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
   }
   lap(c, kPhIcmGrow);
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
