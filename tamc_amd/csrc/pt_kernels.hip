@@ -547,9 +547,9 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   lap(c, kPhIcmSeed);
   // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
   //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing;
-  //     each pass makes two rounds over the sites between barriers: updates made by other waves
-  //     during the first round are already visible in the second, so clusters grow by more than one
-  //     hop per barrier.
+  //     each pass makes several rounds over the sites between barriers (four when sites live in
+  //     registers, two otherwise): updates made by other waves during one round are already
+  //     visible in the next, so clusters grow by more than one hop per barrier.
   __syncthreads();
   constexpr uint32_t kRegSites = 4;
   if (D != 0u && D <= 4u && n <= kRegSites * c.nthreads) {
@@ -573,7 +573,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     for (;;) {
       if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
       int changed = 0;
-      for (uint32_t rep = 0; rep < 2u; ++rep) {
+      for (uint32_t rep = 0; rep < 4u; ++rep) {
 #pragma unroll
         for (uint32_t s = 0; s < kRegSites; ++s) {
           if (cv[s] == qv[s]) continue;  // nothing left to join (also q == 0 and padding)
