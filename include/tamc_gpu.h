@@ -67,6 +67,13 @@ tamc_status tamc_instance_get_info(const tamc_instance *inst, tamc_instance_info
 /* Copy out coupling_vecs in CSR form: row_ptr[n+1], col[nnz], J[nnz], bias[n]. */
 tamc_status tamc_instance_get_csr(const tamc_instance *inst, uint32_t *row_ptr, uint32_t *col, float *J,
                                   float *bias);
+/* Copy out the ICM graph (the coupling CsMat pattern, src/pt.rs:343-346) in CSR form:
+ * row_ptr[graph_nodes+1], col[graph_nnz]. */
+tamc_status tamc_instance_get_graph(const tamc_instance *inst, uint32_t *row_ptr, uint32_t *col);
+/* BqmIsingInstance::induced_subgraph_instance_sorted (src/ising.rs:327-355): the instance induced
+ * by the ascending site list nodes[count]; sites are renumbered 0..count-1, the offset is 0. */
+tamc_status tamc_instance_induced_subgraph(const tamc_instance *inst, const uint32_t *nodes, uint32_t count,
+                                           tamc_instance **out);
 void tamc_instance_destroy(tamc_instance *inst);
 
 /* --------------------------------------------- elementary device functions ---- */
@@ -283,6 +290,48 @@ tamc_status tamc_write_anneal_state(const tamc_sa_result *r, const char *path);
 tamc_status tamc_write_thermal_samples_pickle(const tamc_pt_result *r, const float *betas, uint8_t compression_level,
                                               const char *path);
 tamc_status tamc_write_anneal_state_pickle(const tamc_sa_result *r, const char *path);
+
+/* ------------------------------------------------------------------- GLA ---- */
+/* Graph partitions for GLA (GlaParams::read_graph_partition, src/gla.rs:35-59): one partition per
+ * line of whitespace-separated site indices (read_u32_lines, src/util.rs:142-172: a line that does
+ * not start with an index is skipped; reading stops at the first token that is not one).  Fails
+ * with TAMC_E_INVALID_ARG on an index >= n or an index listed twice, as the reference does.
+ * Partition p holds sites[offsets[p] .. offsets[p+1]), sorted ascending (src/gla.rs:88-91). */
+typedef struct {
+  uint32_t num_partitions;
+  uint32_t *offsets; /* [num_partitions + 1] */
+  uint32_t *sites;   /* [offsets[num_partitions]] */
+} tamc_partition;
+tamc_status tamc_partition_from_file(const tamc_instance *inst, const char *path, tamc_partition **out);
+/* The same checks on caller arrays (partition p = sites[offsets[p] .. offsets[p+1]), any order). */
+tamc_status tamc_partition_create(const tamc_instance *inst, uint32_t num_partitions, const uint32_t *offsets,
+                                  const uint32_t *sites, tamc_partition **out);
+void tamc_partition_free(tamc_partition *p);
+
+/* run_gla (src/gla.rs:122-196): every partition's induced sub-instance is solved by PT-ICM with the
+ * same parameters, all in one batched launch (sub-solve k uses seed params->seed + k); each
+ * sub-solve's last ground state is taken, and the couplings between partitions are summed over the
+ * edges of the partition graph (src/gla.rs:73-120, 175-190), in the reference's edge order.
+ * Mirrors GlaResults (src/gla.rs:62-70); final_state stays empty as in the reference, the joined
+ * state is returned beside it. */
+typedef struct {
+  uint32_t instance_size, num_partitions, num_boundaries;
+  float *partition_energies;  /* [num_partitions] last ground-state energy of each sub-solve */
+  uint32_t *boundary_pairs;   /* [num_boundaries][2] (source partition, target partition) */
+  float *boundary_energies;   /* [num_boundaries] sum of K s_i s_j over the edges between them */
+  float bulk_energy;          /* sum of partition_energies (f32, in order) */
+  float boundary_energy;      /* sum of boundary_energies (f32, in order) */
+  float total_energy;         /* bulk_energy + boundary_energy */
+  int8_t *joined_state;       /* [instance_size] the sub-solves' ground states joined (+-1) */
+  uint64_t attempts;          /* Metropolis attempts of all sub-solves */
+  double device_ms;           /* kernel time of the batched launch */
+} tamc_gla_result;
+tamc_status tamc_gla_run(const tamc_instance *inst, const tamc_partition *parts, const tamc_pt_params *params,
+                         tamc_gla_result **out);
+void tamc_gla_result_free(tamc_gla_result *r);
+/* GlaResults as write_data writes it (src/util.rs:26-36): pickle when the path ends in .pkl, else
+ * bincode-1. */
+tamc_status tamc_write_gla_results(const tamc_gla_result *r, const char *path);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,17 @@ class ServerStats(C.Structure):
                 ("attempts", C.c_uint64), ("pool_used", C.c_uint64), ("pool_bytes", C.c_uint64)]
 
 
+class Partition(C.Structure):
+    _fields_ = [("num_partitions", C.c_uint32), ("offsets", _u32p), ("sites", _u32p)]
+
+
+class GlaResult(C.Structure):
+    _fields_ = [("instance_size", C.c_uint32), ("num_partitions", C.c_uint32), ("num_boundaries", C.c_uint32),
+                ("partition_energies", _f32p), ("boundary_pairs", _u32p), ("boundary_energies", _f32p),
+                ("bulk_energy", C.c_float), ("boundary_energy", C.c_float), ("total_energy", C.c_float),
+                ("joined_state", _i8p), ("attempts", C.c_uint64), ("device_ms", C.c_double)]
+
+
 _vp = C.c_void_p
 _vpp = C.POINTER(C.c_void_p)
 SIGNATURES = {
@@ -108,6 +119,8 @@ SIGNATURES = {
     "tamc_instance_add_suscept_file": (C.c_int, [_vp, C.c_char_p]),
     "tamc_instance_get_info": (C.c_int, [_vp, C.POINTER(InstanceInfo)]),
     "tamc_instance_get_csr": (C.c_int, [_vp, _u32p, _u32p, _f32p, _f32p]),
+    "tamc_instance_get_graph": (C.c_int, [_vp, _u32p, _u32p]),
+    "tamc_instance_induced_subgraph": (C.c_int, [_vp, _u32p, C.c_uint32, _vpp]),
     "tamc_instance_destroy": (None, [_vp]),
     "tamc_energy": (C.c_int, [_vp, C.c_uint32, _i8p, _f32p]),
     "tamc_delta_energy": (C.c_int, [_vp, C.c_uint32, _i8p, _f32p]),
@@ -139,6 +152,12 @@ SIGNATURES = {
     "tamc_write_anneal_state": (C.c_int, [C.POINTER(SaResult), C.c_char_p]),
     "tamc_write_thermal_samples_pickle": (C.c_int, [C.POINTER(PtResult), _f32p, C.c_uint8, C.c_char_p]),
     "tamc_write_anneal_state_pickle": (C.c_int, [C.POINTER(SaResult), C.c_char_p]),
+    "tamc_partition_from_file": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.POINTER(Partition))]),
+    "tamc_partition_create": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.POINTER(Partition))]),
+    "tamc_partition_free": (None, [C.POINTER(Partition)]),
+    "tamc_gla_run": (C.c_int, [_vp, C.POINTER(Partition), C.POINTER(PtParams), C.POINTER(C.POINTER(GlaResult))]),
+    "tamc_gla_result_free": (None, [C.POINTER(GlaResult)]),
+    "tamc_write_gla_results": (C.c_int, [C.POINTER(GlaResult), C.c_char_p]),
 }
 
 _lib = None

@@ -15,6 +15,7 @@ import numpy as np
 from ._lib import (BatchOptions, BatchStats, InstanceInfo, PtParams as _CPt, PtResult, RequestTimes, SaParams as _CSa,
                    SaResult, ServerOptions, ServerStats, TAMC_E_PENDING, check, lib, ptr, _f32p, _f64p, _i8p, _i64p,
                    _u32p, _u64p, _u8p)
+from . import _lib
 
 
 class BqmIsingInstance:
@@ -62,6 +63,22 @@ class BqmIsingInstance:
         check(lib().tamc_instance_get_csr(self._h, ptr(row_ptr, _u32p), ptr(col, _u32p), ptr(J, _f32p),
                                           ptr(bias, _f32p)))
         return row_ptr, col[:self.nnz], J[:self.nnz], bias[:self.n]
+
+    def graph(self):
+        """ICM graph (coupling CsMat pattern) as (row_ptr, col)."""
+        info = InstanceInfo()
+        check(lib().tamc_instance_get_info(self._h, C.byref(info)))
+        row_ptr = np.zeros(info.graph_nodes + 1, np.uint32)
+        col = np.zeros(max(info.graph_nnz, 1), np.uint32)
+        check(lib().tamc_instance_get_graph(self._h, ptr(row_ptr, _u32p), ptr(col, _u32p)))
+        return row_ptr, col[:info.graph_nnz]
+
+    def induced_subgraph(self, nodes):
+        """induced_subgraph_instance_sorted (src/ising.rs:327-355) on ascending site indices."""
+        nodes = np.ascontiguousarray(nodes, np.uint32)
+        h = C.c_void_p()
+        check(lib().tamc_instance_induced_subgraph(self._h, ptr(nodes, _u32p), len(nodes), C.byref(h)))
+        return BqmIsingInstance(h.value)
 
     # -- parity surface ------------------------------------------------------------------
     def energy(self, states):
@@ -290,6 +307,74 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
               icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms)
     return res, st
+
+
+@dataclass
+class GlaParams:
+    """GlaParams (src/gla.rs:28-33).  num_threads is accepted for fidelity: the sub-solves all run in
+    one batched launch."""
+    num_threads: int
+    pt_params: "PtIcmParams"
+    partition_file: str = ""
+
+
+def _partition_list(P):
+    offs = _arr(P.offsets, P.num_partitions + 1, np.uint32)
+    sites = _arr(P.sites, int(offs[-1]), np.uint32)
+    return [sites[offs[k]:offs[k + 1]].copy() for k in range(P.num_partitions)]
+
+
+def read_partition(instance, path):
+    """GlaParams::read_graph_partition (src/gla.rs:35-59): the partitions of a partition file, each
+    sorted ascending."""
+    out = C.POINTER(_lib.Partition)()
+    check(lib().tamc_partition_from_file(instance._h, str(path).encode(), C.byref(out)))
+    try:
+        return _partition_list(out.contents)
+    finally:
+        lib().tamc_partition_free(out)
+
+
+def run_gla(instance, pt_params, partitions=None, partition_file=None, output_file=None):
+    """run_gla (src/gla.rs:122-196) on the GPU: the partitions' sub-instances solved in one batched
+    launch (sub-solve k with seed pt_params.seed + k), joined, with boundary energies.  Returns the
+    GlaResults fields plus the joined state; writes them to output_file (bincode, pickle for .pkl)
+    when given."""
+    out = C.POINTER(_lib.Partition)()
+    if partition_file is not None:
+        check(lib().tamc_partition_from_file(instance._h, str(partition_file).encode(), C.byref(out)))
+    else:
+        parts = [np.ascontiguousarray(p, np.uint32) for p in partitions]
+        offs = np.zeros(len(parts) + 1, np.uint32)
+        offs[1:] = np.cumsum([len(p) for p in parts])
+        sites = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(1, np.uint32), np.uint32)
+        check(lib().tamc_partition_create(instance._h, len(parts), ptr(offs, _u32p), ptr(sites, _u32p),
+                                          C.byref(out)))
+    try:
+        betas, lo = pt_params.resolve()
+        P = pt_params.to_c(betas, lo)
+        res = C.POINTER(_lib.GlaResult)()
+        check(lib().tamc_gla_run(instance._h, out, C.byref(P), C.byref(res)))
+    finally:
+        parts_list = _partition_list(out.contents)
+        lib().tamc_partition_free(out)
+    try:
+        R = res.contents
+        nb = R.num_boundaries
+        pairs = _arr(R.boundary_pairs, 2 * nb, np.uint32).reshape(nb, 2)
+        bnd = _arr(R.boundary_energies, nb, np.float32)
+        d = dict(instance_size=R.instance_size, num_partitions=R.num_partitions,
+                 partition_energies=_arr(R.partition_energies, R.num_partitions, np.float32),
+                 boundary_energies=[(int(a), int(b), float(e)) for (a, b), e in zip(pairs, bnd)],
+                 boundary_energies_f32=bnd, final_state=[], total_energy=np.float32(R.total_energy),
+                 bulk_energy=np.float32(R.bulk_energy), boundary_energy=np.float32(R.boundary_energy),
+                 joined_state=_arr(R.joined_state, R.instance_size, np.int8), partitions=parts_list,
+                 attempts=R.attempts, device_ms=R.device_ms)
+        if output_file is not None:
+            check(lib().tamc_write_gla_results(res, str(output_file).encode()))
+        return d
+    finally:
+        lib().tamc_gla_result_free(res)
 
 
 class Server:

@@ -32,7 +32,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 SOURCES = ["instance.hip", "pt_kernels.hip", "pt_runtime.hip", "lockstep.hip", "elementary.hip", "sa.hip",
-           "writers.hip", "server.hip"]
+           "writers.hip", "server.hip", "gla.hip"]
 
 
 def _run(cmd, quiet=False):

@@ -7,7 +7,9 @@
 // PT (src/pt.rs:609-655): output-file gets PtIcmMinResults as YAML; the sample file (default
 // samples.bin) gets PtIcmThermalSamples as bincode, or as a pickle when its name ends in .pkl.
 // SA (src/sa.rs:216-270): output-file gets AnnealMinResults when it ends in .yml or .pkl; the sample
-// file gets the AnnealState of the ground states.  GLA is not part of this engine.
+// file gets the AnnealState of the ground states.
+// GLA (src/gla.rs:122-196): output-file gets GlaResults, bincode or (for .pkl) pickle; the
+// partitions' sub-instances are solved in one batched launch.
 //
 // The reference seeds its generators from the OS; so does this binary, unless TAMC_SEED is set.
 // --dry-run (not in the reference) parses everything and prints the resolved run as JSON without
@@ -317,6 +319,13 @@ struct SaMethod {
   uint32_t num_replicas = 0, threads = 0;
 };
 
+// GlaParams (src/gla.rs:28-33)
+struct GlaMethod {
+  uint32_t num_threads = 0;
+  PtMethod pt;
+  std::string partition_file;
+};
+
 const Node &need(const Node &m, const char *k, const char *ctx) {
   const Node *v = m.get(k);
   if (!v) throw Error(std::string(ctx) + ": missing field `" + k + "`");
@@ -404,6 +413,8 @@ PtMethod parse_pt(const Node &m) {
   return p;
 }
 
+GlaMethod parse_gla(const Node &m);
+
 SaMethod parse_sa(const Node &m) {
   if (m.kind != Node::Map) throw Error("SA: expected a mapping");
   SaMethod s;
@@ -411,6 +422,17 @@ SaMethod parse_sa(const Node &m) {
   s.num_replicas = (uint32_t)as_uint(need(m, "num_replicas", "SA"), "num_replicas", 0xFFFFFFFFull);
   s.threads = (uint32_t)as_uint(need(m, "threads", "SA"), "threads", 0xFFFFFFFFull);
   return s;
+}
+
+GlaMethod parse_gla(const Node &m) {
+  if (m.kind != Node::Map) throw Error("GLA: expected a mapping");
+  GlaMethod g;
+  g.num_threads = (uint32_t)as_uint(need(m, "num_threads", "GLA"), "num_threads", 0xFFFFFFFFull);
+  g.pt = parse_pt(need(m, "pt_params", "GLA"));
+  const Node &f = need(m, "partition_file", "GLA");
+  if (f.kind != Node::Scalar) throw Error("invalid type for `partition_file`, expected a string");
+  g.partition_file = f.s;
+  return g;
 }
 
 // ------------------------------------------------------------------------- number text ----
@@ -868,6 +890,56 @@ int run_sa(const Args &a, const SaMethod &m) {
   return 0;
 }
 
+tamc_pt_params pt_params_of(const PtMethod &p, const std::vector<float> &betas, uint32_t lo) {
+  tamc_pt_params P{};
+  P.num_sweeps = p.num_sweeps;
+  P.warmup_fraction = p.warmup_fraction;
+  P.num_betas = (uint32_t)betas.size();
+  P.betas = betas.data();
+  P.lo_beta_idx = lo;
+  P.icm = p.icm;
+  P.num_replica_chains = p.num_replica_chains;
+  P.threads = p.threads;
+  P.sample_enabled = p.sample.has;
+  P.sample = p.sample.v;
+  P.sample_states_enabled = p.sample_states.has;
+  P.sample_states = p.sample_states.v;
+  P.sample_limiting = p.sample_limiting.has ? p.sample_limiting.v : 0;
+  return P;
+}
+
+// run_gla (src/gla.rs:122-196): the partitions' sub-instances are solved in one batched launch.
+int run_gla(const Args &a, const GlaMethod &g) {
+  tamc_instance *inst = nullptr;
+  check(tamc_instance_from_file(a.instance.c_str(), a.qubo, &inst));
+  std::unique_ptr<tamc_instance, void (*)(tamc_instance *)> guard(inst, tamc_instance_destroy);
+  tamc_partition *parts = nullptr;
+  check(tamc_partition_from_file(inst, g.partition_file.c_str(), &parts));
+  std::unique_ptr<tamc_partition, void (*)(tamc_partition *)> pguard(parts, tamc_partition_free);
+  uint32_t lo = 0;
+  const std::vector<float> betas = resolve_betas(g.pt.beta, g.pt.lo_beta, g.pt.lo_num_beta, g.pt.icm, &lo);
+  if (a.dry_run) {
+    printf("{\"method\": \"GLA\", \"n\": %u, \"num_partitions\": %u, \"num_threads\": %u, \"num_sweeps\": %u, "
+           "\"betas\": %s, \"lo_beta_idx\": %u, \"icm\": %s, \"output\": \"%s\", \"output_format\": \"%s\"}\n",
+           parts->num_partitions ? parts->offsets[parts->num_partitions] : 0u, parts->num_partitions, g.num_threads,
+           g.pt.num_sweeps, json_floats(betas).c_str(), lo, g.pt.icm ? "true" : "false", a.output.c_str(),
+           ends_with(a.output, "pkl") ? "pickle" : "bincode");
+    return 0;
+  }
+  check(tamc_init());
+  info("tamc::gla", " ** Running GLA with PT-ICM sub-solvers ...");
+  tamc_pt_params P = pt_params_of(g.pt, betas, lo);
+  P.seed = run_seed();
+  tamc_gla_result *r = nullptr;
+  check(tamc_gla_run(inst, parts, &P, &r));
+  std::unique_ptr<tamc_gla_result, void (*)(tamc_gla_result *)> rg(r, tamc_gla_result_free);
+  info("tamc::gla", "Bulk energy: " + rust_display(r->bulk_energy, true));
+  info("tamc::gla", "Boundary energy: " + rust_display(r->boundary_energy, true));
+  info("tamc::gla", "Total energy: " + rust_display(r->total_energy, true));
+  check(tamc_write_gla_results(r, a.output.c_str()));
+  return 0;
+}
+
 int run(int argc, char **argv) {
   const Args a = parse_args(argc, argv);
   const std::string text = read_file(a.method);
@@ -883,13 +955,14 @@ int run(int argc, char **argv) {
   try {
     if (tag == "PT") return run_pt(a, parse_pt(doc.map[0].second));
     if (tag == "SA") return run_sa(a, parse_sa(doc.map[0].second));
+    if (tag == "GLA") return run_gla(a, parse_gla(doc.map[0].second));
   } catch (const Error &e) {
     const std::string w = e.what();
-    if (w.rfind("PT:", 0) == 0 || w.rfind("SA:", 0) == 0 || w.rfind("invalid", 0) == 0 || w.rfind("`", 0) == 0)
+    if (w.rfind("PT:", 0) == 0 || w.rfind("SA:", 0) == 0 || w.rfind("GLA:", 0) == 0 || w.rfind("invalid", 0) == 0 ||
+        w.rfind("`", 0) == 0)
       throw Error("Failed to parse method file (" + a.method + "): " + w);
     throw;
   }
-  if (tag == "GLA") throw Error("GLA is not provided by this engine (PT and SA are)");
   throw Error("Failed to parse method file (" + a.method + "): unknown variant `" + tag +
               "`, expected one of `PT`, `SA`, `GLA`");
 }

@@ -416,6 +416,67 @@ tamc_status tamc_instance_get_csr(const tamc_instance *inst, uint32_t *row_ptr, 
   return TAMC_OK;
 }
 
+tamc_status tamc_instance_get_graph(const tamc_instance *inst, uint32_t *row_ptr, uint32_t *col) {
+  clear_error();
+  if (!inst) return fail(TAMC_E_INVALID_ARG, "null argument");
+  if (row_ptr) memcpy(row_ptr, inst->g_row_ptr.data(), inst->g_row_ptr.size() * 4);
+  if (col && !inst->g_col.empty()) memcpy(col, inst->g_col.data(), inst->g_col.size() * 4);
+  return TAMC_OK;
+}
+
+// induced_subgraph_instance_sorted (src/ising.rs:327-355): the sites `nodes` (ascending) become
+// 0..count-1; offset 0; biases, coupling rows (QUBO self entries included) and susceptibility
+// coefficients restricted to them.  The ICM graph is the pattern of the restricted rows, self
+// entries included, as the reference's TriMat of the same triplets.
+tamc_status tamc_instance_induced_subgraph(const tamc_instance *inst, const uint32_t *nodes, uint32_t count,
+                                           tamc_instance **out) {
+  clear_error();
+  if (!inst || !out || (count && !nodes)) return fail(TAMC_E_INVALID_ARG, "null argument");
+  for (uint32_t k = 0; k < count; ++k) {
+    if (nodes[k] >= inst->n) return fail(TAMC_E_INVALID_ARG, "induced subgraph: site index out of range");
+    if (k && nodes[k] <= nodes[k - 1]) return fail(TAMC_E_INVALID_ARG, "induced subgraph: sites must ascend");
+  }
+  for (const auto &sc : inst->chi)
+    if (sc.size() != inst->n) return fail(TAMC_E_INVALID_ARG, "induced subgraph: a susceptibility vector has the wrong length");
+  tamc_instance *I = new tamc_instance();
+  I->n = count;
+  I->qubo = inst->qubo;
+  I->offset = 0.f;
+  I->bias.resize(count);
+  I->row_ptr.assign((size_t)count + 1, 0u);
+  std::vector<std::pair<uint32_t, uint32_t>> pat;
+  for (uint32_t a = 0; a < count; ++a) {
+    const uint32_t ni = nodes[a];
+    I->bias[a] = inst->bias[ni];
+    for (uint32_t e = inst->row_ptr[ni]; e < inst->row_ptr[ni + 1]; ++e) {
+      const uint32_t* hit = std::lower_bound(nodes, nodes + count, inst->col[e]);
+      if (hit == nodes + count || *hit != inst->col[e]) continue;
+      const uint32_t b = (uint32_t)(hit - nodes);
+      I->col.push_back(b);
+      I->J.push_back(inst->J[e]);
+      pat.emplace_back(a, b);
+    }
+    I->row_ptr[a + 1] = (uint32_t)I->col.size();
+  }
+  uint32_t gn = 0;
+  for (auto &e : pat) gn = std::max(gn, std::max(e.first, e.second) + 1u);
+  I->g_nodes = pat.empty() ? 0u : gn;
+  I->g_row_ptr.assign((size_t)I->g_nodes + 1, 0u);
+  for (auto &e : pat) I->g_row_ptr[e.first + 1]++;
+  for (uint32_t r = 0; r < I->g_nodes; ++r) I->g_row_ptr[r + 1] += I->g_row_ptr[r];
+  I->g_col.resize(pat.size());
+  std::vector<uint32_t> fill(I->g_row_ptr.begin(), I->g_row_ptr.end());
+  for (auto &e : pat) I->g_col[fill[e.first]++] = e.second;
+  for (const auto &sc : inst->chi) {
+    std::vector<double> v(count);
+    for (uint32_t a = 0; a < count; ++a) v[a] = sc[nodes[a]];
+    I->chi.push_back(std::move(v));
+  }
+  compute_schedules(I);
+  *out = I;
+  return TAMC_OK;
+}
+
 void tamc_instance_destroy(tamc_instance *inst) { delete inst; }
 
 }  // extern "C"

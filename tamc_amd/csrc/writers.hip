@@ -79,12 +79,14 @@ struct Pickle {
     op('(');
   }
   void list_end() { op('e'); }
+  void tuple3() { op(0x87); }  // TUPLE3: the last three items become a tuple
   void stop() { op('.'); }
 };
 
-tamc_status write_file(const char *path, const std::vector<uint8_t> &b) {
+tamc_status write_file(const char *path, const std::vector<uint8_t> &b,
+                       const char *what = "Failed to create sample output file ") {
   FILE *f = fopen(path, "wb");
-  if (!f) return fail(TAMC_E_IO, std::string("Failed to create sample output file ") + path);
+  if (!f) return fail(TAMC_E_IO, std::string(what) + path);
   const size_t w = b.empty() ? 0 : fwrite(b.data(), 1, b.size(), f);
   fclose(f);
   if (w != b.size()) return fail(TAMC_E_IO, std::string("short write to ") + path);
@@ -248,6 +250,60 @@ tamc_status tamc_write_anneal_state_pickle(const tamc_sa_result *r, const char *
   o.dict_end();
   o.stop();
   return write_file(path, o.b);
+}
+
+
+// GlaResults (src/gla.rs:62-70) through write_data (src/util.rs:26-36): pickle for .pkl paths,
+// else bincode-1 (boundary entries are (u32, u32, f32) tuples, packed in order).
+tamc_status tamc_write_gla_results(const tamc_gla_result *r, const char *path) {
+  clear_error();
+  if (!r || !path) return fail(TAMC_E_INVALID_ARG, "null argument");
+  const size_t len = strlen(path);
+  const bool pkl = len >= 4 && strcmp(path + len - 4, ".pkl") == 0;
+  const char *what = "Failed to write GLA data to ";
+  if (pkl) {
+    Pickle o;
+    o.dict_begin();
+    o.str("instance_size");
+    o.integer(r->instance_size);
+    o.str("num_partitions");
+    o.integer(r->num_partitions);
+    o.str("partition_energies");
+    o.list_begin();
+    for (uint32_t k = 0; k < r->num_partitions; ++k) o.real(r->partition_energies[k]);
+    o.list_end();
+    o.str("boundary_energies");
+    o.list_begin();
+    for (uint32_t k = 0; k < r->num_boundaries; ++k) {
+      o.integer(r->boundary_pairs[2 * k]);
+      o.integer(r->boundary_pairs[2 * k + 1]);
+      o.real(r->boundary_energies[k]);
+      o.tuple3();
+    }
+    o.list_end();
+    o.str("final_state");
+    o.list_begin();
+    o.list_end();
+    o.str("total_energy");
+    o.real(r->total_energy);
+    o.dict_end();
+    o.stop();
+    return write_file(path, o.b, what);
+  }
+  Out o;
+  o.u32(r->instance_size);
+  o.u32(r->num_partitions);
+  o.u64(r->num_partitions);
+  for (uint32_t k = 0; k < r->num_partitions; ++k) o.f32(r->partition_energies[k]);
+  o.u64(r->num_boundaries);
+  for (uint32_t k = 0; k < r->num_boundaries; ++k) {
+    o.u32(r->boundary_pairs[2 * k]);
+    o.u32(r->boundary_pairs[2 * k + 1]);
+    o.f32(r->boundary_energies[k]);
+  }
+  o.u64(0);  // final_state: left empty by the reference
+  o.f32(r->total_energy);
+  return write_file(path, o.b, what);
 }
 
 }  // extern "C"

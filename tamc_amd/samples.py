@@ -4,6 +4,7 @@ scripts (the files themselves follow the reference's layouts, `src/util.rs:26-36
 * `read_thermal_samples`: `PtIcmThermalSamples` (`src/pt.rs:116-125`), bincode (legacy encoding:
   little-endian, u64 lengths) or, for `.pkl` files, the pickle the writer emits.
 * `read_anneal_state`: `AnnealState` (`src/sa.rs:67-72`), bincode or pickle.
+* `read_gla_results`: `GlaResults` (`src/gla.rs:62-70`), bincode or pickle.
 * `unpack_states`: bit-packed states (`src/ising.rs:96-121`: bit 1 <=> spin -1, LSB first,
   ceil(N/8) bytes per state) to +-1 arrays.
 """
@@ -91,6 +92,28 @@ def read_anneal_state(path):
         r = _Reader(f.read())
     out = {"instance_size": r.u32(), "num_replicas": r.u32()}
     out["bytes"] = r.array(np.uint8)
+    r.done()
+    return out
+
+
+def read_gla_results(path):
+    """GlaResults as a dict: instance_size, num_partitions, partition_energies f32[P],
+    boundary_energies [(i, j, f32)], final_state u8[], total_energy f32."""
+    if _is_pickle(path):
+        with open(path, "rb") as f:
+            obj = pickle.load(f)
+        return {"instance_size": int(obj["instance_size"]), "num_partitions": int(obj["num_partitions"]),
+                "partition_energies": np.asarray(obj["partition_energies"], np.float32),
+                "boundary_energies": [(int(a), int(b), np.float32(e)) for a, b, e in obj["boundary_energies"]],
+                "final_state": np.asarray(obj["final_state"], np.uint8),
+                "total_energy": np.float32(obj["total_energy"])}
+    with open(path, "rb") as f:
+        r = _Reader(f.read())
+    out = {"instance_size": r.u32(), "num_partitions": r.u32()}
+    out["partition_energies"] = r.array(np.float32)
+    out["boundary_energies"] = r.vec(lambda: (r.u32(), r.u32(), np.float32(r.take("f")[0])))
+    out["final_state"] = r.array(np.uint8)
+    out["total_energy"] = np.float32(r.take("f")[0])
     r.done()
     return out
 
