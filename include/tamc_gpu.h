@@ -196,6 +196,11 @@ typedef struct {
  * identical to tamc_pt_run on the same inputs.  results[count] receives one result per solve. */
 tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats);
+/* tamc_pt_run_batch with initial states: init_states[k] is null or [C][B][n] spins (+-1) of solve k,
+ * as for tamc_pt_run (src/pt.rs:397-408). */
+tamc_status tamc_pt_run_batch_init(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                                   const int8_t *const *init_states, const tamc_batch_options *opts,
+                                   tamc_pt_result **results, tamc_batch_stats *stats);
 /* The plain path: the same solves, one instance per kernel launch, one after another. */
 tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               tamc_pt_result **results, tamc_batch_stats *stats);
@@ -290,6 +295,19 @@ tamc_status tamc_write_anneal_state(const tamc_sa_result *r, const char *path);
 tamc_status tamc_write_thermal_samples_pickle(const tamc_pt_result *r, const float *betas, uint8_t compression_level,
                                               const char *path);
 tamc_status tamc_write_anneal_state_pickle(const tamc_sa_result *r, const char *path);
+
+/* ------------------------------------------------------- tamc-pt-opt ---- */
+/* One temperature-set update of pt_optimize_beta (src/bin/tamc-pt-opt.rs:57-167), in f32 with the
+ * reference's operation order.  Inputs per instance k: dif_hists[k][nt][2], the runs' diffusion
+ * histograms summed over chains, and round_trips[k], summed over chains.  momentum[nt] is updated
+ * in place; new_betas[nt] receives the next set; err the mean |log10| change; taus[k] the round-trip
+ * times.  *updated = 0 when the eta normalisation is below f32 epsilon: the reference then keeps
+ * the set (new_betas = beta_vec) and leaves the momentum unchanged.  Fails with
+ * TAMC_E_INVALID_ARG where the reference's bisection panics. */
+tamc_status tamc_pt_opt_update(uint32_t nt, const float *beta_vec, float *momentum, uint32_t count,
+                               const uint32_t *dif_hists, const uint32_t *round_trips, uint32_t num_sweeps,
+                               uint32_t num_chains, float alpha, float m, float *new_betas, float *err,
+                               float *taus, int *updated);
 
 /* ------------------------------------------------------------------- GLA ---- */
 /* Graph partitions for GLA (GlaParams::read_graph_partition, src/gla.rs:35-59): one partition per

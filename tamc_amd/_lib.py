@@ -137,6 +137,11 @@ SIGNATURES = {
                                     C.POINTER(C.POINTER(PtResult)), C.POINTER(BatchStats)]),
     "tamc_pt_run_plain": (C.c_int, [C.c_uint32, _vpp, C.POINTER(PtParams), C.POINTER(C.POINTER(PtResult)),
                                     C.POINTER(BatchStats)]),
+    "tamc_pt_run_batch_init": (C.c_int, [C.c_uint32, _vpp, C.POINTER(PtParams), C.POINTER(_i8p),
+                                         C.POINTER(BatchOptions), C.POINTER(C.POINTER(PtResult)),
+                                         C.POINTER(BatchStats)]),
+    "tamc_pt_opt_update": (C.c_int, [C.c_uint32, _f32p, _f32p, C.c_uint32, _u32p, _u32p, C.c_uint32, C.c_uint32,
+                                     C.c_float, C.c_float, _f32p, _f32p, _f32p, C.POINTER(C.c_int)]),
     "tamc_server_create": (C.c_int, [C.POINTER(ServerOptions), _vpp]),
     "tamc_server_submit": (C.c_int, [_vp, _vp, C.POINTER(PtParams), _i8p, _u64p]),
     "tamc_server_poll": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.POINTER(C.POINTER(PtResult)),

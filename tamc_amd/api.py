@@ -285,13 +285,19 @@ def _batch_args(instances, params_list):
 
 
 def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False,
-                 autotune=False):
-    """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path."""
+                 autotune=False, init_states=None):
+    """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path.
+    init_states: optional list (None entries allowed) of [C][B][n] +-1 start states per solve."""
     resolved, cps, hs = _batch_args(instances, params_list)
     outs = (C.POINTER(PtResult) * len(instances))()
     stats = BatchStats()
     if plain:
         check(lib().tamc_pt_run_plain(len(instances), hs, cps, outs, C.byref(stats)))
+    elif init_states is not None:
+        keep = [None if s is None else np.ascontiguousarray(s, np.int8) for s in init_states]
+        ptrs = (_i8p * len(instances))(*[_i8p() if s is None else ptr(s, _i8p) for s in keep])
+        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune))
+        check(lib().tamc_pt_run_batch_init(len(instances), hs, cps, ptrs, C.byref(opt), outs, C.byref(stats)))
     else:
         opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune))
         check(lib().tamc_pt_run_batch(len(instances), hs, cps, C.byref(opt), outs, C.byref(stats)))
@@ -307,6 +313,57 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
               icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms)
     return res, st
+
+
+def pt_opt_update(betas, momentum, dif_hists, round_trips, num_sweeps, num_chains, alpha, m):
+    """One temperature-set update (tamc_pt_opt_update).  Returns (new_betas, new_momentum, err, taus,
+    updated)."""
+    b = np.ascontiguousarray(betas, np.float32)
+    mo = np.array(momentum, np.float32)
+    h = np.ascontiguousarray(dif_hists, np.uint32)
+    r = np.ascontiguousarray(round_trips, np.uint32)
+    out = np.zeros(len(b), np.float32)
+    taus = np.zeros(len(r), np.float32)
+    err = C.c_float()
+    upd = C.c_int()
+    check(lib().tamc_pt_opt_update(len(b), ptr(b, _f32p), ptr(mo, _f32p), len(r), ptr(h, _u32p), ptr(r, _u32p),
+                                   int(num_sweeps), int(num_chains), float(alpha), float(m), ptr(out, _f32p),
+                                   C.byref(err), ptr(taus, _f32p), C.byref(upd)))
+    return out, mo, np.float32(err.value), taus, bool(upd.value)
+
+
+def pt_optimize_beta(instances, params, num_iters, alpha, m, tol, log=None):
+    """pt_optimize_beta (src/bin/tamc-pt-opt.rs:34-198): every iteration runs PT on all instances
+    in one batched launch (iteration i, instance k: seed params.seed + i * count + k) from the previous
+    iteration's final states, then updates the temperature set.  Returns (params with the optimised
+    Arr betas and doubled num_sweeps, tau history [iterations][instances])."""
+    import dataclasses
+    params = dataclasses.replace(params)
+    count = len(instances)
+    betas = params.resolve()[0]
+    momentum = np.array(betas, np.float32)
+    init = [None] * count
+    tau_hist = []
+    for it in range(num_iters):
+        betas = params.resolve()[0]
+        ps = [dataclasses.replace(params, seed=params.seed + it * count + k) for k in range(count)]
+        res, _ = run_pt_batch(instances, ps, init_states=init)
+        hists = np.stack([r["diffusion_hist"].sum(axis=0, dtype=np.uint64).astype(np.uint32) for r in res])
+        rts = np.array([int(r["round_trips"].sum()) for r in res], np.uint32)
+        new, momentum_next, err, taus, updated = pt_opt_update(betas, momentum, hists, rts, params.num_sweeps,
+                                                               params.num_replica_chains, alpha, m)
+        tau_hist.append(taus)
+        if log:
+            log(it, betas, new, err, taus, updated)
+        if not updated:
+            continue
+        momentum = momentum_next
+        params.beta = BetaOptions.Arr([float(x) for x in new])
+        init = [r["final_states"] for r in res]
+        if err < tol:
+            break
+    params.num_sweeps *= 2
+    return params, np.array(tau_hist, np.float32)
 
 
 @dataclass

@@ -20,6 +20,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libtamc_gpu.so")
 CLI = os.path.join(LIB_DIR, "tamc")
+CLI_PT_OPT = os.path.join(LIB_DIR, "tamc-pt-opt")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -32,7 +33,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 SOURCES = ["instance.hip", "pt_kernels.hip", "pt_runtime.hip", "lockstep.hip", "elementary.hip", "sa.hip",
-           "writers.hip", "server.hip", "gla.hip"]
+           "writers.hip", "server.hip", "gla.hip", "pt_opt.hip"]
 
 
 def _run(cmd, quiet=False):
@@ -76,6 +77,8 @@ def build(verbose=True):
         # the command-line front end is plain host C++ over the C ABI
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CLI, cli_src,
               "-L", LIB_DIR, "-ltamc_gpu", "-Wl,-rpath,$ORIGIN"])
+        # tamc-pt-opt is the same program under another name (it dispatches on argv[0])
+        shutil.copy2(CLI, CLI_PT_OPT)
     if verbose:
         print("built", LIB)
     return LIB

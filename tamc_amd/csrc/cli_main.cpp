@@ -528,20 +528,25 @@ void yaml_list(YamlOut &y, int ind, const char *key, const T *v, size_t n, F fmt
 std::string opt_text(const Opt<uint32_t> &o) { return o.has ? std::to_string(o.v) : "~"; }
 
 // PtIcmMinResults (src/pt.rs:96-102, MinResults flattened from src/ising_results.rs:4-13)
+// PtIcmParams (src/pt.rs:259-272) at indentation ind
+void pt_params_yaml(YamlOut &y, int ind, const PtMethod &p) {
+  y.line(ind, "num_sweeps: " + std::to_string(p.num_sweeps));
+  y.line(ind, "warmup_fraction: " + yaml_float(p.warmup_fraction, false));
+  yaml_beta(y, ind, p.beta);
+  y.line(ind, "lo_beta: " + (p.lo_beta.has ? yaml_float(p.lo_beta.v, true) : std::string("~")));
+  y.line(ind, "lo_num_beta: " + opt_text(p.lo_num_beta));
+  y.line(ind, std::string("icm: ") + (p.icm ? "true" : "false"));
+  y.line(ind, "num_replica_chains: " + std::to_string(p.num_replica_chains));
+  y.line(ind, "threads: " + std::to_string(p.threads));
+  y.line(ind, "sample: " + opt_text(p.sample));
+  y.line(ind, "sample_states: " + opt_text(p.sample_states));
+  y.line(ind, "sample_limiting: " + opt_text(p.sample_limiting));
+}
+
 std::string pt_min_results_yaml(const PtMethod &p, const tamc_pt_result &r) {
   YamlOut y;
   y.line(0, "params:");
-  y.line(2, "num_sweeps: " + std::to_string(p.num_sweeps));
-  y.line(2, "warmup_fraction: " + yaml_float(p.warmup_fraction, false));
-  yaml_beta(y, 2, p.beta);
-  y.line(2, "lo_beta: " + (p.lo_beta.has ? yaml_float(p.lo_beta.v, true) : std::string("~")));
-  y.line(2, "lo_num_beta: " + opt_text(p.lo_num_beta));
-  y.line(2, std::string("icm: ") + (p.icm ? "true" : "false"));
-  y.line(2, "num_replica_chains: " + std::to_string(p.num_replica_chains));
-  y.line(2, "threads: " + std::to_string(p.threads));
-  y.line(2, "sample: " + opt_text(p.sample));
-  y.line(2, "sample_states: " + opt_text(p.sample_states));
-  y.line(2, "sample_limiting: " + opt_text(p.sample_limiting));
+  pt_params_yaml(y, 2, p);
   y.line(0, "timing: " + yaml_float(r.timing_us, false));
   yaml_list(y, 0, "gs_time_steps", r.gs_steps, r.n_gs, [](uint32_t v) { return std::to_string(v); });
   yaml_list(y, 0, "gs_energies", r.gs_energies, r.n_gs, [](float v) { return yaml_float(v, true); });
@@ -967,9 +972,185 @@ int run(int argc, char **argv) {
               "`, expected one of `PT`, `SA`, `GLA`");
 }
 
+// ------------------------------------------------------------------ tamc-pt-opt ----
+// src/bin/tamc-pt-opt.rs: optimise the PT temperature set over a set of instances.  Every iteration
+// runs PT on all instances in one batched launch (iteration i, instance k: seed + i * count + k),
+// starting from the previous iteration's final states, and moves the set towards equal diffusion
+// steps (tamc_pt_opt_update).
+struct OptArgs {
+  uint32_t max_iters = 1000;
+  float step_size = 0.2f, momentum = 0.85f, tolerance = 0.02f;
+  std::string opt_params = "opt_params.yml", tau_hist = "tau_hist.csv", params;
+  std::vector<std::string> instances;
+};
+
+OptArgs parse_opt_args(int argc, char **argv) {
+  OptArgs o;
+  std::vector<std::string> pos;
+  auto value = [&](int &k, const std::string &s, const char *name) -> std::string {
+    const size_t eq = s.find('=');
+    if (eq != std::string::npos) return s.substr(eq + 1);
+    if (k + 1 >= argc) throw Error(std::string("The argument '") + name + "' requires a value");
+    return argv[++k];
+  };
+  for (int k = 1; k < argc; ++k) {
+    const std::string s = argv[k];
+    const std::string key = s.substr(0, s.find('='));
+    if (s == "-h" || s == "--help") {
+      printf("pt-opt\nOptimize the PT temperature set over a set of instances.\n\nUSAGE:\n    tamc-pt-opt [OPTIONS] "
+             "<params> [instances]...\n\nOPTIONS:\n        --max-iters <max-iters>       [default: 1000]\n"
+             "        --momentum <momentum>         [default: 0.85]\n        --opt-params <opt-params>     "
+             "Destination for optimized parameters [default: opt_params.yml]\n        --step-size <step-size>       "
+             "[default: 0.2]\n        --tau-hist <tau-hist>         [default: tau_hist.csv]\n        --tolerance "
+             "<tolerance>       Covergence tolerance criterion (MAE of log-betas) [default: 0.02]\n");
+      exit(0);
+    } else if (key == "--max-iters") {
+      o.max_iters = (uint32_t)std::stoul(value(k, s, "--max-iters <max-iters>"));
+    } else if (key == "--step-size") {
+      o.step_size = std::stof(value(k, s, "--step-size <step-size>"));
+    } else if (key == "--momentum") {
+      o.momentum = std::stof(value(k, s, "--momentum <momentum>"));
+    } else if (key == "--tolerance") {
+      o.tolerance = std::stof(value(k, s, "--tolerance <tolerance>"));
+    } else if (key == "--opt-params") {
+      o.opt_params = value(k, s, "--opt-params <opt-params>");
+    } else if (key == "--tau-hist") {
+      o.tau_hist = value(k, s, "--tau-hist <tau-hist>");
+    } else if (s.size() > 1 && s[0] == '-') {
+      throw Error("Found argument '" + s + "' which wasn't expected");
+    } else {
+      pos.push_back(s);
+    }
+  }
+  if (pos.empty()) throw Error("The following required arguments were not provided: <params>");
+  o.params = pos[0];
+  o.instances.assign(pos.begin() + 1, pos.end());
+  return o;
+}
+
+std::string fmt(const char *f, double v) {
+  char b[64];
+  snprintf(b, sizeof b, f, v);
+  return b;
+}
+
+int run_pt_opt(int argc, char **argv) {
+  const OptArgs o = parse_opt_args(argc, argv);
+  std::vector<std::unique_ptr<tamc_instance, void (*)(tamc_instance *)>> insts;
+  std::vector<const tamc_instance *> ptrs;
+  for (auto &f : o.instances) {
+    tamc_instance *inst = nullptr;
+    check(tamc_instance_from_file(f.c_str(), 0, &inst));
+    insts.emplace_back(inst, tamc_instance_destroy);
+    ptrs.push_back(inst);
+  }
+  const Node doc = Yaml(read_file(o.params)).parse();
+  if (doc.kind != Node::Map || doc.map.size() != 1 || doc.map[0].first != "PT") {
+    fprintf(stderr, "WARN  [tamc_pt_opt] Must input a PT method for tamc-pt-opt\n");
+    return 0;
+  }
+  PtMethod p = parse_pt(doc.map[0].second);
+  if (ptrs.empty()) throw Error("no instances given");
+  check(tamc_init());
+  const uint32_t count = (uint32_t)ptrs.size(), nc = p.num_replica_chains;
+  uint32_t lo = 0;
+  std::vector<float> betas = resolve_betas(p.beta, p.lo_beta, p.lo_num_beta, p.icm, &lo);
+  const uint32_t nt = (uint32_t)betas.size();
+  std::vector<float> momentum = betas;
+  std::vector<std::vector<float>> tau_hist;
+  std::vector<std::vector<int8_t>> init(count);
+  const uint64_t seed = run_seed();
+  for (uint32_t it = 0; it < o.max_iters; ++it) {
+    printf("* Iteration %u.\n* Step Size: %s\n", it, rust_display(o.step_size, true).c_str());
+    betas = resolve_betas(p.beta, p.lo_beta, p.lo_num_beta, p.icm, &lo);
+    std::vector<tamc_pt_params> P(count);
+    std::vector<const int8_t *> init_ptrs(count, nullptr);
+    for (uint32_t k = 0; k < count; ++k) {
+      P[k] = pt_params_of(p, betas, lo);
+      P[k].seed = seed + (uint64_t)it * count + k;
+      if (!init[k].empty()) init_ptrs[k] = init[k].data();
+    }
+    std::vector<tamc_pt_result *> res(count, nullptr);
+    tamc_batch_options bo{};
+    tamc_batch_stats bs{};
+    check(tamc_pt_run_batch_init(count, ptrs.data(), P.data(), init_ptrs.data(), &bo, res.data(), &bs));
+    struct Free {
+      std::vector<tamc_pt_result *> &r;
+      ~Free() {
+        for (auto *x : r) tamc_pt_result_free(x);
+      }
+    } fr{res};
+    std::vector<uint32_t> hist((size_t)count * nt * 2, 0u), rts(count, 0u);
+    for (uint32_t k = 0; k < count; ++k) {
+      const tamc_pt_result *r = res[k];
+      for (uint32_t c = 0; c < r->C; ++c) {
+        rts[k] += r->round_trips[c];
+        for (uint32_t b = 0; b < nt * 2; ++b) hist[(size_t)k * nt * 2 + b] += r->diffusion_hist[((size_t)c * nt) * 2 + b];
+      }
+    }
+    std::vector<float> next(nt), taus(count);
+    float err = 0.f;
+    int updated = 0;
+    check(tamc_pt_opt_update(nt, betas.data(), momentum.data(), count, hist.data(), rts.data(), p.num_sweeps, nc,
+                             o.step_size, o.momentum, next.data(), &err, taus.data(), &updated));
+    std::string tl = "Round-trip times (sweeps):\n[";
+    for (uint32_t k = 0; k < count; ++k) tl += (k ? ", " : "") + rust_display(taus[k], true);
+    printf("%s]\n", (tl).c_str());
+    tau_hist.push_back(taus);
+    if (!updated) {
+      fprintf(stderr, "WARN  [tamc_pt_opt]  ** Insufficient round trips for eta CDF\n");
+      continue;
+    }
+    printf("%9s%9s\n--------------------------------------------------\n", "beta0", "beta");
+    for (uint32_t b = 0; b < nt; ++b) printf(" %7.4f  %7.4f \n", betas[b], next[b]);
+    printf("Mean abs log rel_err: %s\n", rust_display(err, true).c_str());
+    p.beta.geometric = false;
+    p.beta.arr = next;
+    for (uint32_t k = 0; k < count; ++k) {
+      const tamc_pt_result *r = res[k];
+      init[k].assign(r->final_states, r->final_states + (size_t)r->C * r->B * r->n);
+    }
+    if (err < o.tolerance) {
+      printf(" ** Relative Error converged\n");
+      break;
+    }
+  }
+  betas = resolve_betas(p.beta, p.lo_beta, p.lo_num_beta, p.icm, &lo);
+  std::string fl = "Final temperature array:\n[";
+  for (uint32_t b = 0; b < nt; ++b) fl += (b ? ", " : "") + fmt("%6.5f", betas[b]);
+  printf("%s]\n", fl.c_str());
+  p.num_sweeps *= 2;
+  YamlOut y;
+  y.line(0, "PT:");
+  pt_params_yaml(y, 2, p);
+  printf("%s\n", y.s.c_str());
+  printf("\n\n ** Writing to %s **\n", o.opt_params.c_str());
+  write_text(o.opt_params, y.s, "optimized parameters");
+  std::string csv;
+  for (auto &row : tau_hist) {
+    for (float x : row) csv += rust_display(x, true) + ",";
+    csv += "\n";
+  }
+  write_text(o.tau_hist, csv, "tau history");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
+  {
+    const char *prog = strrchr(argv[0], '/');
+    prog = prog ? prog + 1 : argv[0];
+    if (strcmp(prog, "tamc-pt-opt") == 0) {
+      try {
+        return run_pt_opt(argc, argv);
+      } catch (const std::exception &e) {
+        fflush(stdout);
+        fprintf(stderr, "tamc-pt-opt terminated with an error:\n%s\n", e.what());
+        return 1;
+      }
+    }
+  }
   try {
     const int rc = run(argc, argv);
     bool dry = false;

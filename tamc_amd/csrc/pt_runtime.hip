@@ -688,6 +688,12 @@ static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *in
 
 tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                               const tamc_batch_options *opts, tamc_pt_result **results, tamc_batch_stats *stats) {
+  return tamc_pt_run_batch_init(count, insts, params, nullptr, opts, results, stats);
+}
+
+tamc_status tamc_pt_run_batch_init(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                                   const int8_t *const *init_states, const tamc_batch_options *opts,
+                                   tamc_pt_result **results, tamc_batch_stats *stats) {
   clear_error();
   const auto t_entry = std::chrono::steady_clock::now();
   if (!insts || !params || !results) return fail(TAMC_E_INVALID_ARG, "null argument");
@@ -710,7 +716,8 @@ tamc_status tamc_pt_run_batch(uint32_t count, const tamc_instance *const *insts,
   opt.t_entry = t_entry;  // the autotuner's probe launches count as preparation
   std::vector<SolvePlan> solves(count);
   for (uint32_t k = 0; k < count; ++k)
-    if ((st = plan_solve(solves[k], insts[k], &params[k], nullptr, opt.waves))) return st;
+    if ((st = plan_solve(solves[k], insts[k], &params[k], init_states ? init_states[k] : nullptr, opt.waves)))
+      return st;
   return run_solves(solves, opt, results, stats);
 }
 
