@@ -309,6 +309,18 @@ tamc_status tamc_pt_opt_update(uint32_t nt, const float *beta_vec, float *moment
                                uint32_t num_chains, float alpha, float m, float *new_betas, float *err,
                                float *taus, int *updated);
 
+/* ----------------------------------------------------------- percolation ---- */
+/* Site percolation (random_percolation, src/percolation.rs:11-75; the percolate binary,
+ * src/bin/percolate.rs): num_samples independent samples on the graph of an instance read in Ising
+ * form (edges = couplings, node weights = diagonal entries).  In sample k site i is occupied with
+ * probability p: rand's Bernoulli rule (v < p * 2^64, always for p = 1) on a 64-bit draw from
+ * Philox4x32-10 keyed by seed, counter {k, i, 0, 4}.  max_cluster[k] = size of the largest
+ * connected cluster of occupied sites (0 when none is occupied, where the reference panics);
+ * percolates[k] = 1 when an occupied site of negative weight is connected to an occupied site of
+ * positive weight (the reference's depth-first search finds a path).  At most 16384 sites. */
+tamc_status tamc_percolate(const tamc_instance *inst, double p, uint32_t num_samples, uint64_t seed,
+                           uint32_t *max_cluster, uint8_t *percolates);
+
 /* ------------------------------------------------------------------- GLA ---- */
 /* Graph partitions for GLA (GlaParams::read_graph_partition, src/gla.rs:35-59): one partition per
  * line of whitespace-separated site indices (read_u32_lines, src/util.rs:142-172: a line that does

@@ -160,6 +160,7 @@ SIGNATURES = {
     "tamc_partition_from_file": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.POINTER(Partition))]),
     "tamc_partition_create": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.POINTER(Partition))]),
     "tamc_partition_free": (None, [C.POINTER(Partition)]),
+    "tamc_percolate": (C.c_int, [_vp, C.c_double, C.c_uint32, C.c_uint64, _u32p, _u8p]),
     "tamc_gla_run": (C.c_int, [_vp, C.POINTER(Partition), C.POINTER(PtParams), C.POINTER(C.POINTER(GlaResult))]),
     "tamc_gla_result_free": (None, [C.POINTER(GlaResult)]),
     "tamc_write_gla_results": (C.c_int, [C.POINTER(GlaResult), C.c_char_p]),

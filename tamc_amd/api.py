@@ -366,6 +366,15 @@ def pt_optimize_beta(instances, params, num_iters, alpha, m, tol, log=None):
     return params, np.array(tau_hist, np.float32)
 
 
+def random_percolation(instance, p, num_samples, seed=0):
+    """random_percolation (src/percolation.rs:11-75) for num_samples samples on the GPU: returns
+    (max_cluster u32[num_samples], percolates bool[num_samples])."""
+    cl = np.zeros(max(num_samples, 1), np.uint32)
+    pc = np.zeros(max(num_samples, 1), np.uint8)
+    check(lib().tamc_percolate(instance._h, float(p), int(num_samples), int(seed), ptr(cl, _u32p), ptr(pc, _u8p)))
+    return cl[:num_samples], pc[:num_samples].astype(bool)
+
+
 @dataclass
 class GlaParams:
     """GlaParams (src/gla.rs:28-33).  num_threads is accepted for fidelity: the sub-solves all run in

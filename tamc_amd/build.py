@@ -21,6 +21,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libtamc_gpu.so")
 CLI = os.path.join(LIB_DIR, "tamc")
 CLI_PT_OPT = os.path.join(LIB_DIR, "tamc-pt-opt")
+CLI_PERCOLATE = os.path.join(LIB_DIR, "percolate")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -33,7 +34,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 SOURCES = ["instance.hip", "pt_kernels.hip", "pt_runtime.hip", "lockstep.hip", "elementary.hip", "sa.hip",
-           "writers.hip", "server.hip", "gla.hip", "pt_opt.hip"]
+           "writers.hip", "server.hip", "gla.hip", "pt_opt.hip", "percolation.hip"]
 
 
 def _run(cmd, quiet=False):
@@ -77,8 +78,9 @@ def build(verbose=True):
         # the command-line front end is plain host C++ over the C ABI
         _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CLI, cli_src,
               "-L", LIB_DIR, "-ltamc_gpu", "-Wl,-rpath,$ORIGIN"])
-        # tamc-pt-opt is the same program under another name (it dispatches on argv[0])
+        # tamc-pt-opt and percolate are the same program under other names (it dispatches on argv[0])
         shutil.copy2(CLI, CLI_PT_OPT)
+        shutil.copy2(CLI, CLI_PERCOLATE)
     if verbose:
         print("built", LIB)
     return LIB

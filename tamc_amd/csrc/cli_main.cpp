@@ -1135,12 +1135,83 @@ int run_pt_opt(int argc, char **argv) {
   return 0;
 }
 
+// ------------------------------------------------------------------ percolate ----
+// src/bin/percolate.rs: site percolation samples of an instance file's graph.
+int run_percolate(int argc, char **argv) {
+  uint32_t samples = 100;
+  double prob = -1.0;
+  bool has_prob = false;
+  std::string output, instance;
+  bool has_output = false, has_instance = false;
+  for (int k = 1; k < argc; ++k) {
+    const std::string s = argv[k];
+    auto value = [&](const char *name) -> std::string {
+      const size_t eq = s.find('=');
+      if (s.rfind("--", 0) == 0 && eq != std::string::npos) return s.substr(eq + 1);
+      if (k + 1 >= argc) throw Error(std::string("The argument '") + name + "' requires a value");
+      return argv[++k];
+    };
+    const std::string key = s.substr(0, s.find('='));
+    if (s == "-h" || s == "--help") {
+      printf("percolate\nPerform a site percolation calculation on an arbitrary graph.\n\nUSAGE:\n    percolate "
+             "[OPTIONS] <instance> --prob <prob>\n\nOPTIONS:\n    -n, --num-samples <num-samples>     [default: 100]\n"
+             "    -o, --output <output>               \n    -p, --prob <prob>                   \n");
+      exit(0);
+    } else if (key == "-n" || key == "--num-samples") {
+      samples = (uint32_t)std::stoul(value("--num-samples <num-samples>"));
+    } else if (key == "-p" || key == "--prob") {
+      prob = std::stod(value("--prob <prob>"));
+      has_prob = true;
+    } else if (key == "-o" || key == "--output") {
+      output = value("--output <output>");
+      has_output = true;
+    } else if (s.size() > 1 && s[0] == '-') {
+      throw Error("Found argument '" + s + "' which wasn't expected");
+    } else if (!has_instance) {
+      instance = s;
+      has_instance = true;
+    } else {
+      throw Error("Found argument '" + s + "' which wasn't expected");
+    }
+  }
+  if (!has_prob || !has_instance)
+    throw Error("The following required arguments were not provided: --prob <prob> <instance>");
+  tamc_instance *inst = nullptr;
+  check(tamc_instance_from_file(instance.c_str(), 0, &inst));
+  std::unique_ptr<tamc_instance, void (*)(tamc_instance *)> guard(inst, tamc_instance_destroy);
+  check(tamc_init());
+  std::vector<uint32_t> cl(std::max<uint32_t>(samples, 1));
+  std::vector<uint8_t> pc(std::max<uint32_t>(samples, 1));
+  check(tamc_percolate(inst, prob, samples, run_seed(), cl.data(), pc.data()));
+  uint32_t num = 0;
+  for (uint32_t k = 0; k < samples; ++k) num += pc[k];
+  printf("# * Percolation results *\n");
+  printf("samps,  num_perc,  frac_perc\n");
+  printf("%u, %u, %s\n", samples, num, rust_display((double)num / (double)samples, false).c_str());
+  if (has_output) {
+    // the reference writes the header and then every sample's largest cluster with no separator
+    std::string text = "perc, max_cluster";
+    for (uint32_t k = 0; k < samples; ++k) text += std::to_string(cl[k]);
+    write_text(output, text, "output file");
+  }
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
   {
     const char *prog = strrchr(argv[0], '/');
     prog = prog ? prog + 1 : argv[0];
+    if (strcmp(prog, "percolate") == 0) {
+      try {
+        return run_percolate(argc, argv);
+      } catch (const std::exception &e) {
+        fflush(stdout);
+        fprintf(stderr, "percolate terminated with an error:\n%s\n", e.what());
+        return 1;
+      }
+    }
     if (strcmp(prog, "tamc-pt-opt") == 0) {
       try {
         return run_pt_opt(argc, argv);

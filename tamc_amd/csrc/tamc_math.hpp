@@ -29,7 +29,7 @@ constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
 constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
 
 // Stream tags carried in the fourth counter word.
-enum : uint32_t { kTagSweep = 0, kTagExchange = 1, kTagIcm = 2, kTagInit = 3 };
+enum : uint32_t { kTagSweep = 0, kTagExchange = 1, kTagIcm = 2, kTagInit = 3, kTagPercolation = 4 };
 
 struct U4 {
   uint32_t x, y, z, w;
