@@ -176,6 +176,27 @@ def other_configs(with_cpu):
         rec["cpu_1thread"] = {"value": o["attempts"] / cdt, "cores": 1, "sample": "1 instance x 64 restarts x %d sweeps" % S}
         rec["vs_cpu_1thread"] = rec["value"] / rec["cpu_1thread"]["value"]
     out["sa_qubo"] = rec
+
+    # the largest request shapes of the serving stream, one instance alone on the device (serving
+    # parameters: 32 betas 0.2-5.0, 2 chains, ICM), timed over 100 sweeps: the solve latency floor
+    # of the stream's tail, with the kernel's per-phase split
+    shapes = {"2d_L128_pmJ": W.lattice_2d(128, np.random.default_rng(8), pm_j=True),
+              "3d_L24_gauss": W.lattice_3d(24, np.random.default_rng(7)),
+              "rr3_N16384_gauss": W.random_regular3(16384, np.random.default_rng(5), gaussian=True)}
+    names = ["icm_overlap", "icm_seed", "icm_growth", "icm_flip", "exchange", "sweep", "measure"]
+    largest = {}
+    for name, (i, j, K, n) in shapes.items():
+        inst = api.BqmIsingInstance.from_lines(i, j, K)
+        T = 100
+        p = api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 32),
+                            lo_beta=1.0, icm=True, num_replica_chains=2, sample=32, sample_states=32,
+                            sample_limiting=2, seed=1)
+        api.run_pt_batch([inst], [p], waves_per_group=16)  # warm (instance upload)
+        _, st = api.run_pt_batch([inst], [p], waves_per_group=16)
+        largest[name] = {"n": n, "us_per_sweep": st["device_ms"] * 1e3 / T,
+                         "phases_us_per_sweep": {k: v * 1e3 / T for k, v in zip(names, st["phase_ms"])},
+                         "icm_passes_per_sweep": st["icm_passes"] / T}
+    out["largest_serving_shapes"] = largest
     return out
 
 

@@ -428,6 +428,185 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   }
 }
 
+// ICM step (c) with every thread's sites in registers: thread tid owns sites tid + s * nthreads
+// (s < S) and is the only writer of their cluster words, so it keeps the cluster word of each in a
+// register, and for S == 4 with rows of width <= 4 also q and the padded neighbour row (otherwise
+// these are re-read each round, beside each other).  RW is the row length in 64-bit words (four
+// u16 entries each; padding entries hold the site itself and are not read).  With re-read rows a
+// round takes the sites four at a time: it gathers the neighbours' cluster words of the unfinished
+// ones, so those loads are in flight together, then joins and writes back; with rows in registers
+// every site is joined and written before the next is read.  Four rounds per barrier.
+template <uint32_t S, uint32_t RW, bool kLds>
+__device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t D) {
+  constexpr bool kRegs = S == 4u && RW == 1u;
+  constexpr uint32_t kR = kRegs ? S : 1u;
+  constexpr uint32_t kChunk = kRegs ? 1u : 4u;
+  constexpr uint32_t kWin = kRegs ? 1u : (S * RW < 12u ? S * RW : 12u);
+  // q is read-only here: a constant view lets its loads move past the cluster-word stores
+  using QPtr = std::conditional_t<kLds, const TAMC_AS3 uint32_t *, const TAMC_AS4 uint32_t *>;
+  const QPtr qw = (QPtr)M.qw;
+  uint32_t cv[S], qv[kR];
+  uint64_t rw[kR];
+  auto row = [&](uint32_t i, uint32_t k) -> uint64_t {
+    return M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * i + k]
+                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[RW * i + k];
+  };
+  // cluster words of the neighbours in one row word (its first `cnt` entries, cnt >= 3)
+  auto gather = [&](uint64_t w, uint32_t cnt) -> uint32_t {
+    uint32_t x = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu];
+    if (cnt >= 4u) x |= M.cw[w >> 48];
+    return x;
+  };
+#pragma unroll
+  for (uint32_t s = 0; s < S; ++s) {
+    const uint32_t i = c.tid + s * c.nthreads;
+    uint32_t q = 0u, cw = 0u;
+    uint64_t r = 0ull;
+    if (i < n) {
+      cw = M.cw[i];
+      if (kRegs) {
+        q = qw[i];
+        r = row(i, 0u);
+      }
+    }
+    cv[s] = cw;
+    if (kRegs) {
+      qv[s % kR] = q;
+      rw[s % kR] = r;
+    }
+  }
+  for (;;) {
+    if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+    int changed = 0;
+#pragma unroll 1
+    for (uint32_t rep = 0; rep < 4u; ++rep) {
+      uint32_t base = c.tid;
+      asm volatile("" : "+v"(base));  // keeps the site addresses out of registers between rounds
+      // re-read rows stream through a window of kWin row words: the first kWin are requested at the
+      // start of the round and each slot is refilled with the word kWin further on as soon as it is
+      // read, so the loads stay ahead of their use; sites past n read row 0 and are never joined
+      // (q = 0)
+      uint64_t pre[kWin];
+      auto fetch = [&](uint32_t wd) {  // row word wd of the round: site wd / RW, word wd % RW
+        const uint32_t i = base + (wd / RW) * c.nthreads;
+        pre[wd % kWin] = row(i < n ? i : 0u, wd % RW);
+      };
+      if (!kRegs) {
+#pragma unroll
+        for (uint32_t wd = 0; wd < kWin; ++wd) fetch(wd);
+      }
+#pragma unroll
+      for (uint32_t s0 = 0; s0 < S; s0 += kChunk) {
+        uint32_t nb[kChunk], qk[kChunk];
+#pragma unroll
+        for (uint32_t k = 0; k < kChunk; ++k) {
+          const uint32_t s = s0 + k, i = base + s * c.nthreads;
+          uint32_t q = 0u;
+          uint64_t w0 = 0ull, w1 = 0ull;
+          if (kRegs) {
+            q = qv[s % kR];
+            w0 = rw[s % kR];
+          } else {
+            if (i < n) q = qw[i];
+            w0 = pre[(s * RW) % kWin];
+            if (RW == 2u) w1 = pre[(s * RW + 1u) % kWin];
+#pragma unroll
+            for (uint32_t k2 = 0; k2 < RW; ++k2)
+              if (s * RW + k2 + kWin < S * RW) fetch(s * RW + k2 + kWin);
+          }
+          uint32_t x = 0u;
+          if (cv[s] != q) {  // something left to join (never for q == 0 and padding)
+            if (RW == 1u) {
+              x = gather(w0, D);
+            } else {
+              x = gather(w0, 4u) | M.cw[w1 & 0xFFFFu];
+              if (D >= 6u) x |= M.cw[(w1 >> 16) & 0xFFFFu];
+              if (D >= 7u) x |= M.cw[(w1 >> 32) & 0xFFFFu];
+              if (D == 8u) x |= M.cw[w1 >> 48];
+            }
+          }
+          nb[k] = x;
+          qk[k] = q;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kChunk; ++k) {
+          const uint32_t s = s0 + k;
+          const uint32_t nc = cv[s] | (qk[k] & nb[k]);
+          if (nc != cv[s]) {
+            M.cw[base + s * c.nthreads] = nc;
+            changed = 1;
+          }
+          cv[s] = nc;
+        }
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+}
+
+// ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster
+// when its q bit p is set and a neighbour is in it.  Passes update in place until a pass changes
+// nothing; each pass makes several rounds over the sites between barriers (four when the sites
+// live in registers, two otherwise): updates made by other waves during one round are already
+// visible in the next, so clusters grow by more than one hop per barrier.  The cluster fixpoint
+// does not depend on the visiting order.
+template <bool kLds>
+__device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t n = c.n, D = c.nbr ? X.ell_D : 0u;
+  if (D != 0u && D <= 4u && n <= 4u * c.nthreads) {
+    icm_grow_registers<4, 1>(c, M, n, D);
+  } else if (D != 0u && D <= 4u && n <= 8u * c.nthreads) {
+    icm_grow_registers<8, 1>(c, M, n, D);
+  } else if (D != 0u && D <= 4u && n <= 16u * c.nthreads) {
+    icm_grow_registers<16, 1>(c, M, n, D);
+  } else if (D > 4u && D <= 8u && n <= 4u * c.nthreads) {
+    icm_grow_registers<4, 2>(c, M, n, D);
+  } else if (D > 4u && D <= 8u && n <= 8u * c.nthreads) {
+    icm_grow_registers<8, 2>(c, M, n, D);
+  } else if (D > 4u && D <= 8u && n <= 16u * c.nthreads) {
+    icm_grow_registers<16, 2>(c, M, n, D);
+  } else {
+    for (;;) {
+      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+      int changed = 0;
+      for (uint32_t rep = 0; rep < 2u; ++rep)
+      for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+        const uint32_t q = M.qw[i];
+        const uint32_t cur = M.cw[i];
+        if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
+        uint32_t nb = 0;
+        if (D != 0u && D <= 4u) {
+          // one 8-byte load of the (padded) row
+          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
+          nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu] | M.cw[w >> 48];
+        } else if (D != 0u) {
+          uint64_t w0, w1;
+          if (M.nbr_lds) {
+            w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i];
+            w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i + 1u];
+          } else {
+            w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i];
+            w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i + 1u];
+          }
+          nb = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu] | M.cw[w0 >> 48] |
+               M.cw[w1 & 0xFFFFu] | M.cw[(w1 >> 16) & 0xFFFFu] | M.cw[(w1 >> 32) & 0xFFFFu] | M.cw[w1 >> 48];
+        } else if (i < X.g_nodes) {
+          const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
+          for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
+        }
+        const uint32_t nc = cur | (q & nb);
+        if (nc != cur) {
+          M.cw[i] = nc;
+          changed = 1;
+        }
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+  }
+}
+
 // 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
 // the two chains; bit p of the per-site words qw/cw belongs to pair p.
 template <bool kLds>
@@ -545,89 +724,32 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     if (c.lane == 0) c.seed_blk[p] = sb;
   }
   lap(c, kPhIcmSeed);
-  // (c) grow every pair's cluster through its q<0 sites: a site joins pair p's cluster when its q bit
-  //     p is set and a neighbour is in it.  Passes update in place until a pass changes nothing;
-  //     each pass makes several rounds over the sites between barriers (four when sites live in
-  //     registers, two otherwise): updates made by other waves during one round are already
-  //     visible in the next, so clusters grow by more than one hop per barrier.
   __syncthreads();
-  constexpr uint32_t kRegSites = 4;
-  if (D != 0u && D <= 4u && n <= kRegSites * c.nthreads) {
-    // every thread owns at most four sites and is the only writer of their cluster words, so it
-    // keeps q, the cluster word and the neighbour row of each in registers: a visit reads only the
-    // neighbours' cluster words (three for rows of width 3, whose fourth entry is the site itself)
-    uint32_t qv[kRegSites], cv[kRegSites];
-    uint64_t rw[kRegSites];
-#pragma unroll
-    for (uint32_t s = 0; s < kRegSites; ++s) {
-      const uint32_t i = c.tid + s * c.nthreads;
-      qv[s] = cv[s] = 0u;
-      rw[s] = 0ull;
-      if (i < n) {
-        qv[s] = M.qw[i];
-        cv[s] = M.cw[i];
-        rw[s] = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
-                          : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
-      }
-    }
-    for (;;) {
-      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
-      int changed = 0;
-      for (uint32_t rep = 0; rep < 4u; ++rep) {
-#pragma unroll
-        for (uint32_t s = 0; s < kRegSites; ++s) {
-          if (cv[s] == qv[s]) continue;  // nothing left to join (also q == 0 and padding)
-          const uint64_t w = rw[s];
-          uint32_t nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu];
-          if (D == 4u) nb |= M.cw[w >> 48];
-          const uint32_t nc = cv[s] | (qv[s] & nb);
-          if (nc != cv[s]) {
-            cv[s] = nc;
-            M.cw[c.tid + s * c.nthreads] = nc;
-            changed = 1;
-          }
-        }
-      }
-      if (!__syncthreads_or(changed)) break;
-    }
+  if constexpr (kLds) {
+    icm_grow(c, M);
   } else {
-    for (;;) {
-      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
-      int changed = 0;
-      for (uint32_t rep = 0; rep < 2u; ++rep)
-      for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-        const uint32_t q = M.qw[i];
-        const uint32_t cur = M.cw[i];
-        if (cur == q) continue;  // no q<0 bit left to join (also q == 0)
-        uint32_t nb = 0;
-        if (D != 0u && D <= 4u) {
-          // one 8-byte load of the (padded) row
-          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
-                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
-          nb = M.cw[w & 0xFFFFu] | M.cw[(w >> 16) & 0xFFFFu] | M.cw[(w >> 32) & 0xFFFFu] | M.cw[w >> 48];
-        } else if (D != 0u) {
-          uint64_t w0, w1;
-          if (M.nbr_lds) {
-            w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i];
-            w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[2u * i + 1u];
-          } else {
-            w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i];
-            w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[2u * i + 1u];
-          }
-          nb = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu] | M.cw[w0 >> 48] |
-               M.cw[w1 & 0xFFFFu] | M.cw[(w1 >> 16) & 0xFFFFu] | M.cw[(w1 >> 32) & 0xFFFFu] | M.cw[w1 >> 48];
-        } else if (i < X.g_nodes) {
-          const uint32_t e0 = c4(X.g_row_ptr)[i], e1 = c4(X.g_row_ptr)[i + 1];
-          for (uint32_t e = e0; e < e1; ++e) nb |= M.cw[c4(X.g_col)[e]];
-        }
-        const uint32_t nc = cur | (q & nb);
-        if (nc != cur) {
-          M.cw[i] = nc;
-          changed = 1;
-        }
-      }
-      if (!__syncthreads_or(changed)) break;
+    // the spins are not read while clusters grow: park the planes in the group's spin array and
+    // grow in the LDS they leave (q and cluster words take 8 bytes per site, the planes 8 G)
+    const uint32_t nw = n * G * 2u;
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
+    __syncthreads();
+    IcmMem<true> S;
+    S.qw = (TAMC_AS3 uint32_t *)c.sp;
+    S.cw = (TAMC_AS3 uint32_t *)(c.sp + n);
+    S.blk = nullptr;
+    S.nbr_lds = M.nbr_lds;
+    S.nbr_glb = M.nbr_glb;
+    for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+      S.qw[i] = M.qw[i];
+      S.cw[i] = M.cw[i];
     }
+    __syncthreads();
+    icm_grow(c, S);
+    __syncthreads();
+    for (uint32_t i = c.tid; i < n; i += c.nthreads) M.cw[i] = S.cw[i];
+    __syncthreads();
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
+    __syncthreads();
   }
   lap(c, kPhIcmGrow);
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change

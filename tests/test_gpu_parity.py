@@ -125,6 +125,31 @@ def test_pt_production_large_instances(gpu, kind, kw, exact):
     assert_same(g, o, exact_energy=exact)
 
 
+def test_icm_growth_every_sites_per_thread(gpu):
+    # one wave per work group (64 threads): the cluster growth keeps 4, 8 or 16 sites per thread in
+    # registers, for neighbour rows of one 64-bit word (width 4) and of two (3D lattices, width 6)
+    cases = [("2d_pm", dict(L=12)), ("rr3_pm", dict(n=400)), ("2d_pm", dict(L=30)),
+             ("3d_pm", dict(L=6)), ("3d_pm", dict(L=7)), ("3d_pm", dict(L=10))]
+    B, C, T = 8, 2, 12
+    insts, params, oracles = [], [], []
+    for k, (kind, kw) in enumerate(cases):
+        oi, gi, _ = make_pair(kind, seed=40 + k, **kw)
+        insts.append(gi)
+        oracles.append(oi)
+        params.append(api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, B),
+                                      lo_num_beta=4, icm=True, num_replica_chains=C, threads=1, sample=2,
+                                      sample_states=3, sample_limiting=2, seed=900 + k))
+    assert [g.n for g in insts] == [144, 400, 900, 216, 343, 1000]
+    batch, st = api.run_pt_batch(insts, params, waves_per_group=1)
+    assert st["waves_per_group"] == 1
+    for k, (oi, p) in enumerate(zip(oracles, params)):
+        runner = api.PtIcmRunner(insts[k], p)
+        o = O.pt_run(oi, runner.betas, T, num_chains=C, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=True,
+                     threaded=False, sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
+        assert o["error"] == 0
+        assert_same(batch[k], o, exact_energy=True)
+
+
 def test_pt_large_ladder_two_words(gpu):
     # 48 betas x 2 chains = 96 replicas: two bit-plane words per site
     g, o = _run_both("3d_pm", dict(L=3), B=48, lo=28, T=24, sample=3, sstates=5)
