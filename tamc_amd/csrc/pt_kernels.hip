@@ -215,6 +215,17 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 
 // value of v in lane `src` (0..63); every lane of the wave must be active
 // Inclusive running maximum over the 64 lanes (same DPP pattern as wave_incl_scan; 0 is neutral).
+// OR of v over the wave, in every lane
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 __device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
@@ -428,6 +439,98 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   }
 }
 
+// ICM step (c) for the pairs whose clusters still grow after kLabelAfterPasses passes (long
+// clusters take one bit-parallel round per hop): each such pair's cluster is completed as the
+// connected component, in the pair's q<0 sites, of the sites already in it.  Components come from
+// label hooking with shortcutting (Shiloach-Vishkin style), in place: the low 16 bits of the
+// site's q word become its label (0xFFFF outside the subgraph), bits 16.. keep the q bits of the
+// straggling pairs (q of the other pairs is no longer read).  Labels only decrease and always
+// name a site of the same component.  A hook round makes every subgraph edge whose ends carry
+// different labels lower the larger label's site (a root) to the smaller label; a shortcut round
+// then points every label at its root.  When a hook round finds nothing to do, labels are constant
+// on components, and the cluster is every site carrying the label of a site already in it.
+constexpr uint32_t kLabelAfterPasses = 24u, kMaxLabelPairs = 16u, kNoLabel = 0xFFFFu;
+
+template <uint32_t S, uint32_t RW, bool kLds>
+__device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t pairs) {
+  auto row = [&](uint32_t i, uint32_t k) -> uint64_t {
+    return M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * i + k]
+                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[RW * i + k];
+  };
+  for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+    const uint32_t q = M.qw[i];
+    uint32_t hi = 0u, k = 0u;
+    for (uint32_t m = pairs; m; m &= m - 1u, ++k) hi |= ((q >> __builtin_ctz(m)) & 1u) << k;
+    M.qw[i] = (hi << 16) | kNoLabel;
+  }
+  __syncthreads();
+  uint32_t k = 0u;
+  for (uint32_t m = pairs; m; m &= m - 1u, ++k) {
+    const uint32_t p = __builtin_ctz(m), bit = 1u << (16u + k);
+    for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+      const uint32_t w = M.qw[i];
+      M.qw[i] = (w & 0xFFFF0000u) | ((w & bit) ? i : kNoLabel);
+    }
+    if (c.tid == 0u) c.misc[11] = kNoLabel;
+    __syncthreads();
+    for (;;) {
+      if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+      int hooked = 0;
+      // sites four at a time, their rows requested together
+#pragma unroll 1
+      for (uint32_t s0 = 0; s0 < S; s0 += 4u) {
+        uint64_t nb[4][RW];
+#pragma unroll
+        for (uint32_t t = 0; t < 4u; ++t) {
+          const uint32_t i = c.tid + (s0 + t) * c.nthreads, ii = i < n ? i : 0u;
+#pragma unroll
+          for (uint32_t r = 0; r < RW; ++r) nb[t][r] = row(ii, r);
+        }
+#pragma unroll
+        for (uint32_t t = 0; t < 4u; ++t) {
+          const uint32_t i = c.tid + (s0 + t) * c.nthreads;
+          const uint32_t w = i < n ? M.qw[i] : 0u;
+          if (w & bit) {
+            const uint32_t lu = w & 0xFFFFu;
+            uint32_t lmin = lu;
+#pragma unroll
+            for (uint32_t r = 0; r < RW; ++r)
+#pragma unroll
+              for (uint32_t e = 0; e < 4u; ++e) lmin = min(lmin, M.qw[(nb[t][r] >> (16u * e)) & 0xFFFFu] & 0xFFFFu);
+            if (lmin < lu) {
+              // the high bits of a site's word do not change, so the minimum of the words is the
+              // minimum of the labels
+              atomicMin((uint32_t *)&M.qw[lu], (M.qw[lu] & 0xFFFF0000u) | lmin);
+              hooked = 1;
+            }
+          }
+        }
+      }
+      if (!__syncthreads_or(hooked)) break;
+      for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+        const uint32_t w = M.qw[i];
+        if (w & bit) {
+          const uint32_t l0 = w & 0xFFFFu;
+          uint32_t l = l0, up;
+          while ((up = M.qw[l] & 0xFFFFu) != l) l = up;
+          if (l != l0) M.qw[i] = (w & 0xFFFF0000u) | l;
+        }
+      }
+      __syncthreads();
+    }
+    // every site already in the cluster now carries the component's label
+    for (uint32_t i = c.tid; i < n; i += c.nthreads)
+      if ((M.cw[i] >> p) & 1u) c.misc[11] = M.qw[i] & 0xFFFFu;
+    __syncthreads();
+    const uint32_t root = c.misc[11];
+    for (uint32_t i = c.tid; i < n; i += c.nthreads) {
+      const uint32_t w = M.qw[i];
+      if ((w & bit) && (w & 0xFFFFu) == root) M.cw[i] |= 1u << p;
+    }
+    __syncthreads();
+  }
+}
+
 // ICM step (c) with every thread's sites in registers: thread tid owns sites tid + s * nthreads
 // (s < S) and is the only writer of their cluster words, so it keeps the cluster word of each in a
 // register, and for S == 4 with rows of width <= 4 also q and the padded neighbour row (otherwise
@@ -475,9 +578,14 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
       rw[s % kR] = r;
     }
   }
-  for (;;) {
+  // pass k ORs the pairs that gained sites into misc[8 + k % 3]; the slot of pass k - 1 is
+  // cleared after pass k's barrier (it is next written in pass k + 2)
+  if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
+  __syncthreads();
+  uint32_t label_pairs = 0u;
+  for (uint32_t pass = 0;; ++pass) {
     if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
-    int changed = 0;
+    uint32_t gain = 0u;
 #pragma unroll 1
     for (uint32_t rep = 0; rep < 4u; ++rep) {
       uint32_t base = c.tid;
@@ -534,14 +642,25 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
           const uint32_t nc = cv[s] | (qk[k] & nb[k]);
           if (nc != cv[s]) {
             M.cw[base + s * c.nthreads] = nc;
-            changed = 1;
+            gain |= nc ^ cv[s];
           }
           cv[s] = nc;
         }
       }
     }
-    if (!__syncthreads_or(changed)) break;
+    const uint32_t g = wave_or(gain);
+    uint32_t *slot = c.misc + 8u + pass % 3u;
+    if (c.lane == 0u && g) atomicOr(slot, g);
+    __syncthreads();
+    const uint32_t grew = *slot;
+    if (c.tid == 0u) c.misc[8u + (pass + 2u) % 3u] = 0u;
+    if (grew == 0u) break;
+    if (pass + 1u >= kLabelAfterPasses && (uint32_t)__popc(grew) <= kMaxLabelPairs) {
+      label_pairs = grew;
+      break;
+    }
   }
+  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs);
 }
 
 // ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster

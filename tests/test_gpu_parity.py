@@ -76,6 +76,39 @@ def test_houdayer_flip_sets(gpu, kind, kw):
         assert np.array_equal(cl[p], ocl)
 
 
+@pytest.mark.parametrize("long_pairs", [(3, 17, 33), tuple(range(20)) + (35,)])
+def test_houdayer_long_clusters(gpu, long_pairs):
+    # q<0 on a serpentine through a periodic 32x32 lattice: the even rows but the last, joined
+    # through one site of each odd row, alternately at columns 0 and 16, so that the cluster is
+    # 527 sites whose far end is 150 to 270 hops from the seed.  With at most 16 such pairs among
+    # 32 the clusters are finished by component labelling after the first 24 passes; with 20 the
+    # bit-parallel growth goes on until no more than 16 are still growing.  Both must give the
+    # reference's flip sets.
+    L = 32
+    oi, gi, _ = make_pair("2d_pm", seed=9, L=L)
+    rng = np.random.default_rng(13)
+    npairs = 40
+    S1 = random_states(rng, npairs, oi.n)
+    S2 = random_states(rng, npairs, oi.n)
+    r, c = np.divmod(np.arange(oi.n), L)
+    snake = (r < L - 1) & ((r % 2 == 0) | (c == np.where((r // 2) % 2 == 0, 0, L // 2)))
+    for p in long_pairs:
+        S2[p] = np.where(snake, -S1[p], S1[p])
+    ks = []
+    for a, b in zip(S1, S2):
+        cnt = O.num_negative_overlap(a, b)
+        ks.append(int(rng.integers(0, cnt)) if cnt else 0)
+    g1, g2, sizes, cl = gi.houdayer(S1, S2, ks)
+    for p in range(npairs):
+        a, b = S1[p].copy(), S2[p].copy()
+        rr, ocl = oi.houdayer_k(a, b, ks[p])
+        assert sizes[p] == max(rr, 0)
+        assert np.array_equal(g1[p], a) and np.array_equal(g2[p], b)
+        assert np.array_equal(cl[p], ocl)
+    for p in long_pairs:
+        assert sizes[p] == snake.sum()
+
+
 def _run_both(kind, kw, B=8, C=2, T=60, lo=4, threads=1, seed=5, icm=True, sample=4, sstates=4, limiting=0,
               bmin=0.2, bmax=5.0, order="colour"):
     oi, gi, _ = make_pair(kind, seed=seed, **kw)
