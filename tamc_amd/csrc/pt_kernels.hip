@@ -273,11 +273,13 @@ struct IcmMem {
 // first replica's energy change
 //   dE_a(p) = sum_{i in cluster p} -2 s_a(i) (h_i + sum_{j not in cluster p} J_ij s_a(j));
 // the second replica's change is -dE_a (the move conserves E_a + E_b).  A wave takes 64 sites at
-// a time; their (site, pair) memberships are compacted over the lanes (prefix sum of the cluster
-// words' popcounts), so every lane computes one term of a cluster per round, and the terms are
-// added into the pair's fixed-point total.  Then lane = site applies the site's flip mask.  Pair p
-// only reads replica a_p at neighbours outside cluster p and only flips replicas a_p, b_p at sites
-// inside it, so a block is flipped as soon as its terms are done.
+// a time, lane = site.  A lane in some cluster loads its site's data once (bias, row, couplings,
+// the neighbours' cluster words and, with one plane word per site, the plane words of the site
+// and its neighbours), then steps through its pairs one per iteration, adding each term into the
+// pair's fixed-point total.  Then the lane applies the site's flip mask.  Pair p only reads
+// replica a_p at neighbours outside cluster p and only flips replicas a_p, b_p at sites inside
+// it, so a block is flipped as soon as its terms are done, and plane words loaded before other
+// blocks' flips still hold every bit a term reads.
 template <int D, bool kLds>
 __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
@@ -285,84 +287,50 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   // replica indices of every pair, lane p holding pair p
   const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
   const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
-  // one round: lane L takes the (base+L)-th (site, pair) membership of block blk
-  // per-site data of a block, lane = site, loaded once per block and handed to the item lanes
-  struct SiteData {
-    uint32_t bias, nbr_lo, nbr_hi, J[D > 0 ? D : 1];
-  };
-  auto site_data = [&](uint32_t blk) {
-    SiteData sd{};
+  constexpr int kD = D > 0 ? D : 1;
+  // the terms of the pairs in `pmask` for the sites of block blk
+  auto terms = [&](uint32_t blk, uint32_t cwi, uint32_t pmask) {
     const uint32_t i = blk * 64u + c.lane;
-    if (i < n) {
-      sd.bias = __float_as_uint(c4(X.bias)[i]);
-      if constexpr (D > 0 && D <= 4) {
-        const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
-                                     : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
-        sd.nbr_lo = (uint32_t)w;
-        sd.nbr_hi = (uint32_t)(w >> 32);
-      }
+    uint32_t rem = cwi & pmask;
+    if (!ballot(rem != 0u)) return;
+    float bias = 0.0f, J[kD];
+    uint32_t nj[kD], cwj[kD];
+    uint64_t wi = 0ull, wj[kD];
+    if (rem) {
+      bias = c4(X.bias)[i];
       if constexpr (D > 0) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) sd.J[k] = __float_as_uint(c4(X.ell_J)[(size_t)i * D + k]);
-      }
-    }
-    return sd;
-  };
-  auto round = [&](uint32_t blk, uint32_t cwi, uint32_t excl, uint32_t total, uint32_t base, const SiteData &sd) {
-    const uint32_t m = base + c.lane;
-    // the item's site is the last lane s with excl_s <= m.  Every site whose range starts inside
-    // the round posts (excl_s, s) at its start position; a running maximum carries it over the
-    // site's items.  The site covering item `base` (the last with excl_s <= base, non-empty since
-    // base < total) seeds position 0.
-    uint32_t *buf = c.fscr + 64u * c.wave;
-    buf[c.lane] = 0u;
-    if (cwi != 0u && excl >= base && excl < base + 64u) buf[excl - base] = (excl << 8) | c.lane;
-    // lanes read what other lanes wrote: fence at wavefront scope so that the compiler neither
-    // forwards a lane's own zero nor moves the read
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t s0 = 63u - (uint32_t)__builtin_clzll(ballot(excl <= base));
-    uint32_t v = buf[c.lane];
-    if (c.lane == 0u) v = max(v, ((uint32_t)__builtin_amdgcn_readlane((int)excl, (int)s0) << 8) | s0);
-    v = wave_incl_max(v);
-    const uint32_t s = v & 0xFFu, start = v >> 8;
-    const uint32_t cws = lane_value(cwi, s);
-    const float bias_s = __uint_as_float(lane_value(sd.bias, s));
-    uint32_t nlo = 0u, nhi = 0u;
-    float nJ[D > 0 ? D : 1];
-    if constexpr (D > 0) {
-      if constexpr (D <= 4) {
-        nlo = lane_value(sd.nbr_lo, s);
-        nhi = lane_value(sd.nbr_hi, s);
-      }
-#pragma unroll
-      for (int k = 0; k < D; ++k) nJ[k] = __uint_as_float(lane_value(sd.J[k], s));
-    }
-    if (m < total) {
-      const uint32_t p = select_bit(cws, m - start);
-      const uint32_t a = c.pair_a[p];
-      const uint32_t i = blk * 64u + s;
-      float hh = bias_s;
-      if constexpr (D > 0) {
-        uint32_t nj[D];
         if constexpr (D <= 4) {
-          const uint64_t w = ((uint64_t)nhi << 32) | nlo;
+          const uint64_t w = M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[i]
+                                       : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[i];
 #pragma unroll
           for (int k = 0; k < D; ++k) nj[k] = (uint32_t)(w >> (16 * k)) & 0xFFFFu;
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) nj[k] = M.nbr((size_t)i * nbr_stride(D) + k);
         }
-        uint32_t out[D], sj[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          out[k] = ((M.cw[nj[k]] >> p) & 1u) ^ 1u;
-          sj[k] = sbit(c.sp, G, nj[k], a);
+          J[k] = c4(X.ell_J)[(size_t)i * D + k];
+          cwj[k] = M.cw[nj[k]];
+          if (G == 1u) wj[k] = *reinterpret_cast<const uint64_t *>(c.sp + 2u * nj[k]);
         }
+      }
+      if (G == 1u) wi = *reinterpret_cast<const uint64_t *>(c.sp + 2u * i);
+    }
+    while (rem) {
+      const uint32_t p = (uint32_t)__builtin_ctz(rem);
+      rem &= rem - 1u;
+      const uint32_t a = c.pair_a[p];
+      float hh = bias;
+      uint32_t si;
+      if constexpr (D > 0) {
 #pragma unroll
-        for (int k = 0; k < D; ++k)
-          if (out[k]) hh += sj[k] ? -nJ[k] : nJ[k];
+        for (int k = 0; k < D; ++k) {
+          if (!((cwj[k] >> p) & 1u)) {
+            const uint32_t sj = (G == 1u) ? (uint32_t)(wj[k] >> a) & 1u : sbit(c.sp, G, nj[k], a);
+            hh += sj ? -J[k] : J[k];
+          }
+        }
       } else {
         const uint32_t e0 = c4(X.row_ptr)[i], e1 = c4(X.row_ptr)[i + 1];
         for (uint32_t e = e0; e < e1; ++e) {
@@ -373,7 +341,8 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
           }
         }
       }
-      const float term = sbit(c.sp, G, i, a) ? 2.0f * hh : -2.0f * hh;
+      si = (G == 1u) ? (uint32_t)(wi >> a) & 1u : sbit(c.sp, G, i, a);
+      const float term = si ? 2.0f * hh : -2.0f * hh;
       atomicAdd((unsigned long long *)&c.dEa[p], (unsigned long long)to_fix(term));
     }
   };
@@ -407,28 +376,19 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
     for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
       const uint32_t i_lane = blk * 64u + c.lane;
       const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
-      const uint32_t cnt = (uint32_t)__popc(cwi);
-      const uint32_t incl = wave_incl_scan(cnt);
-      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-      if (total == 0u) continue;
-      const SiteData sd = site_data(blk);
-      for (uint32_t base = 0; base < total; base += 64u) round(blk, cwi, incl - cnt, total, base, sd);
+      if (!ballot(cwi != 0u)) continue;
+      terms(blk, cwi, 0xFFFFFFFFu);
       flip_block(blk, cwi);
     }
   } else {
-    // fewer blocks than waves: every wave scans every block and the rounds go round-robin over
-    // the waves; the flips wait until every round is done
-    uint32_t unit = 0;
+    // fewer blocks than waves: every wave takes every block, for the pairs p = wave (mod W); the
+    // flips wait until every term is done
+    uint32_t pmask = 0u;
+    for (uint32_t p = c.wave; p < P; p += c.W) pmask |= 1u << p;
     for (uint32_t blk = 0; blk < c.nblk; ++blk) {
       const uint32_t i_lane = blk * 64u + c.lane;
       const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
-      const uint32_t cnt = (uint32_t)__popc(cwi);
-      const uint32_t incl = wave_incl_scan(cnt);
-      const uint32_t total = __builtin_amdgcn_readlane(incl, 63);
-      if (total == 0u) continue;
-      const SiteData sd = site_data(blk);
-      for (uint32_t base = 0; base < total; base += 64u, ++unit)
-        if (unit % c.W == c.wave) round(blk, cwi, incl - cnt, total, base, sd);
+      terms(blk, cwi, pmask);
     }
     __syncthreads();
     for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
