@@ -493,7 +493,9 @@ __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, u
 
 // ICM step (c) with every thread's sites in registers: thread tid owns sites tid + s * nthreads
 // (s < S) and is the only writer of their cluster words, so it keeps the cluster word of each in a
-// register, and for S == 4 with rows of width <= 4 also q and the padded neighbour row (otherwise
+// register (up to 8 sites; with 16 the words are re-read beside q, which keeps the kernel's
+// register budget for the other phases), and for S == 4 with rows of width <= 4 also q and the
+// padded neighbour row (otherwise
 // these are re-read each round, beside each other).  RW is the row length in 64-bit words (four
 // u16 entries each; padding entries hold the site itself and are not read).  With re-read rows a
 // round takes the sites four at a time: it gathers the neighbours' cluster words of the unfinished
@@ -505,10 +507,12 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
   constexpr uint32_t kR = kRegs ? S : 1u;
   constexpr uint32_t kChunk = kRegs ? 1u : 4u;
   constexpr uint32_t kWin = kRegs ? 1u : (S * RW < 12u ? S * RW : 12u);
+  constexpr bool kCvRegs = S <= 8u;  // with 16 sites the cluster words are re-read from LDS
+  constexpr uint32_t kC = kCvRegs ? S : 1u;
   // q is read-only here: a constant view lets its loads move past the cluster-word stores
   using QPtr = std::conditional_t<kLds, const TAMC_AS3 uint32_t *, const TAMC_AS4 uint32_t *>;
   const QPtr qw = (QPtr)M.qw;
-  uint32_t cv[S], qv[kR];
+  uint32_t cv[kC], qv[kR];
   uint64_t rw[kR];
   auto row = [&](uint32_t i, uint32_t k) -> uint64_t {
     return M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * i + k]
@@ -532,7 +536,7 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
         r = row(i, 0u);
       }
     }
-    cv[s] = cw;
+    if (kCvRegs) cv[s % kC] = cw;
     if (kRegs) {
       qv[s % kR] = q;
       rw[s % kR] = r;
@@ -565,7 +569,7 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
       }
 #pragma unroll
       for (uint32_t s0 = 0; s0 < S; s0 += kChunk) {
-        uint32_t nb[kChunk], qk[kChunk];
+        uint32_t nb[kChunk], qk[kChunk], ck[kChunk];
 #pragma unroll
         for (uint32_t k = 0; k < kChunk; ++k) {
           const uint32_t s = s0 + k, i = base + s * c.nthreads;
@@ -582,8 +586,9 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
             for (uint32_t k2 = 0; k2 < RW; ++k2)
               if (s * RW + k2 + kWin < S * RW) fetch(s * RW + k2 + kWin);
           }
+          const uint32_t cur = kCvRegs ? cv[s % kC] : (i < n ? M.cw[i] : 0u);
           uint32_t x = 0u;
-          if (cv[s] != q) {  // something left to join (never for q == 0 and padding)
+          if (cur != q) {  // something left to join (never for q == 0 and padding)
             if (RW == 1u) {
               x = gather(w0, D);
             } else {
@@ -595,16 +600,17 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
           }
           nb[k] = x;
           qk[k] = q;
+          ck[k] = cur;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kChunk; ++k) {
           const uint32_t s = s0 + k;
-          const uint32_t nc = cv[s] | (qk[k] & nb[k]);
-          if (nc != cv[s]) {
+          const uint32_t nc = ck[k] | (qk[k] & nb[k]);
+          if (nc != ck[k]) {
             M.cw[base + s * c.nthreads] = nc;
-            gain |= nc ^ cv[s];
+            gain |= nc ^ ck[k];
           }
-          cv[s] = nc;
+          if (kCvRegs) cv[s % kC] = nc;
         }
       }
     }
