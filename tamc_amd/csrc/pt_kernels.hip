@@ -166,7 +166,7 @@ struct Ctx {
   bool nbr_in_lds;
   long long *dEa;
   unsigned long long *ticks;  // LDS phase timers [kPhaseSlots] and the last timestamp
-  uint32_t *fscr;             // per-wave 64-word scratch of the ICM flip
+  uint32_t *fscr;             // ICM labelling: [p] a site of pair p's partial cluster
   unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [k][v] = replicas of
                               // the pairs 4k + j with bit j of v set (one plane word per site)
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
@@ -401,66 +401,93 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
 
 // ICM step (c) for the pairs whose clusters still grow after kLabelAfterPasses passes (long
 // clusters take one bit-parallel round per hop): each such pair's cluster is completed as the
-// connected component, in the pair's q<0 sites, of the sites already in it.  Components come from
-// label hooking with shortcutting (Shiloach-Vishkin style), in place: the low 16 bits of the
-// site's q word become its label (0xFFFF outside the subgraph), bits 16.. keep the q bits of the
-// straggling pairs (q of the other pairs is no longer read).  Labels only decrease and always
-// name a site of the same component.  A hook round makes every subgraph edge whose ends carry
-// different labels lower the larger label's site (a root) to the smaller label; a shortcut round
-// then points every label at its root.  When a hook round finds nothing to do, labels are constant
-// on components, and the cluster is every site carrying the label of a site already in it.
-constexpr uint32_t kLabelAfterPasses = 24u, kMaxLabelPairs = 16u, kNoLabel = 0xFFFFu;
+// connected component, in the pair's q<0 sites, of the sites already in it.
+//  * The q words are no longer read, so they hold the labels of two pairs at a time (16 bits
+//    each, 0xFFFF outside the pair's subgraph); the cluster-word bit of a straggling pair holds
+//    the site's q bit until the pair's component is known, and one site of each partial cluster
+//    is kept.
+//  * Components come from label hooking with shortcutting (Shiloach-Vishkin style).  Labels only
+//    decrease and always name a site of the same component.  A hook round makes every subgraph
+//    edge whose ends carry different labels lower the larger label's site (a root) to the smaller
+//    label; a shortcut round then points every label at its root.  When a hook round finds
+//    nothing to do, labels are constant on components, and the cluster is every member site with
+//    the label of the kept site.
+constexpr uint32_t kLabelAfterPasses = 12u, kMaxLabelPairs = 16u, kNoLabel = 0xFFFFu;
+
+// lowers the 16-bit half `h` (0 low, 1 high) of an LDS word to at most l
+template <class U32P>
+__device__ __forceinline__ void min_half(U32P w, uint32_t h, uint32_t l) {
+  const uint32_t sh = 16u * h, mask = 0xFFFFu << sh;
+  uint32_t old = *w;
+  while (((old >> sh) & 0xFFFFu) > l) {
+    const uint32_t seen = atomicCAS((uint32_t *)w, old, (old & ~mask) | (l << sh));
+    if (seen == old) break;
+    old = seen;
+  }
+}
 
 template <uint32_t S, uint32_t RW, bool kLds>
 __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t pairs) {
+  constexpr uint32_t kLabelChunk = (RW == 1u && S >= 8u) ? 8u : 4u;
   auto row = [&](uint32_t i, uint32_t k) -> uint64_t {
     return M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * i + k]
                      : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[RW * i + k];
   };
+  uint32_t *keep = c.fscr;  // [p]: a site of pair p's partial cluster
+  if (c.tid < 32u) keep[c.tid] = 0xFFFFFFFFu;
+  __syncthreads();
   for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-    const uint32_t q = M.qw[i];
-    uint32_t hi = 0u, k = 0u;
-    for (uint32_t m = pairs; m; m &= m - 1u, ++k) hi |= ((q >> __builtin_ctz(m)) & 1u) << k;
-    M.qw[i] = (hi << 16) | kNoLabel;
+    const uint32_t cw = M.cw[i], q = M.qw[i];
+    for (uint32_t m = cw & pairs; m; m &= m - 1u) atomicMin(&keep[__builtin_ctz(m)], i);
+    M.cw[i] = (cw & ~pairs) | (q & pairs);
   }
   __syncthreads();
-  uint32_t k = 0u;
-  for (uint32_t m = pairs; m; m &= m - 1u, ++k) {
-    const uint32_t p = __builtin_ctz(m), bit = 1u << (16u + k);
+  for (uint32_t m = pairs; m;) {
+    const uint32_t pa = (uint32_t)__builtin_ctz(m);
+    m &= m - 1u;
+    const uint32_t pb = m ? (uint32_t)__builtin_ctz(m) : 32u;  // 32: no second pair
+    if (m) m &= m - 1u;
+    const uint32_t ma = 1u << pa, mb = pb < 32u ? 1u << pb : 0u;
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-      const uint32_t w = M.qw[i];
-      M.qw[i] = (w & 0xFFFF0000u) | ((w & bit) ? i : kNoLabel);
+      const uint32_t cw = M.cw[i];
+      M.qw[i] = ((cw & ma) ? i : kNoLabel) | (((cw & mb) ? i : kNoLabel) << 16);
     }
-    if (c.tid == 0u) c.misc[11] = kNoLabel;
     __syncthreads();
     for (;;) {
       if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
       int hooked = 0;
-      // sites four at a time, their rows requested together
+      // sites kLabelChunk at a time, their rows requested together
 #pragma unroll 1
-      for (uint32_t s0 = 0; s0 < S; s0 += 4u) {
-        uint64_t nb[4][RW];
+      for (uint32_t s0 = 0; s0 < S; s0 += kLabelChunk) {
+        uint64_t nb[kLabelChunk][RW];
 #pragma unroll
-        for (uint32_t t = 0; t < 4u; ++t) {
+        for (uint32_t t = 0; t < kLabelChunk; ++t) {
           const uint32_t i = c.tid + (s0 + t) * c.nthreads, ii = i < n ? i : 0u;
 #pragma unroll
           for (uint32_t r = 0; r < RW; ++r) nb[t][r] = row(ii, r);
         }
 #pragma unroll
-        for (uint32_t t = 0; t < 4u; ++t) {
+        for (uint32_t t = 0; t < kLabelChunk; ++t) {
           const uint32_t i = c.tid + (s0 + t) * c.nthreads;
-          const uint32_t w = i < n ? M.qw[i] : 0u;
-          if (w & bit) {
-            const uint32_t lu = w & 0xFFFFu;
-            uint32_t lmin = lu;
+          const uint32_t w = i < n ? M.qw[i] : 0xFFFFFFFFu;
+          if (w != 0xFFFFFFFFu) {
+            const uint32_t la = w & 0xFFFFu, lb = w >> 16;
+            uint32_t mina = la, minb = lb;
 #pragma unroll
             for (uint32_t r = 0; r < RW; ++r)
 #pragma unroll
-              for (uint32_t e = 0; e < 4u; ++e) lmin = min(lmin, M.qw[(nb[t][r] >> (16u * e)) & 0xFFFFu] & 0xFFFFu);
-            if (lmin < lu) {
-              // the high bits of a site's word do not change, so the minimum of the words is the
-              // minimum of the labels
-              atomicMin((uint32_t *)&M.qw[lu], (M.qw[lu] & 0xFFFF0000u) | lmin);
+              for (uint32_t e = 0; e < 4u; ++e) {
+                const uint32_t x = M.qw[(nb[t][r] >> (16u * e)) & 0xFFFFu];
+                mina = min(mina, x & 0xFFFFu);
+                minb = min(minb, x >> 16);
+              }
+            // only a member hooks (a non-member's label 0xFFFF names no site)
+            if (la != kNoLabel && mina < la) {
+              min_half(&M.qw[la], 0u, mina);
+              hooked = 1;
+            }
+            if (lb != kNoLabel && minb < lb) {
+              min_half(&M.qw[lb], 1u, minb);
               hooked = 1;
             }
           }
@@ -469,23 +496,25 @@ __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, u
       if (!__syncthreads_or(hooked)) break;
       for (uint32_t i = c.tid; i < n; i += c.nthreads) {
         const uint32_t w = M.qw[i];
-        if (w & bit) {
-          const uint32_t l0 = w & 0xFFFFu;
-          uint32_t l = l0, up;
-          while ((up = M.qw[l] & 0xFFFFu) != l) l = up;
-          if (l != l0) M.qw[i] = (w & 0xFFFF0000u) | l;
-        }
+        if (w == 0xFFFFFFFFu) continue;
+        uint32_t la = w & 0xFFFFu, lb = w >> 16, up;
+        if (la != kNoLabel)
+          while ((up = M.qw[la] & 0xFFFFu) != la) la = up;
+        if (lb != kNoLabel)
+          while ((up = M.qw[lb] >> 16) != lb) lb = up;
+        const uint32_t nw = la | (lb << 16);
+        if (nw != w) M.qw[i] = nw;
       }
       __syncthreads();
     }
-    // every site already in the cluster now carries the component's label
-    for (uint32_t i = c.tid; i < n; i += c.nthreads)
-      if ((M.cw[i] >> p) & 1u) c.misc[11] = M.qw[i] & 0xFFFFu;
-    __syncthreads();
-    const uint32_t root = c.misc[11];
+    const uint32_t ra = M.qw[keep[pa]] & 0xFFFFu;
+    const uint32_t rb = mb ? M.qw[keep[pb]] >> 16 : kNoLabel;
     for (uint32_t i = c.tid; i < n; i += c.nthreads) {
-      const uint32_t w = M.qw[i];
-      if ((w & bit) && (w & 0xFFFFu) == root) M.cw[i] |= 1u << p;
+      const uint32_t w = M.qw[i], cw = M.cw[i];
+      uint32_t out = cw & ~(ma | mb);
+      if ((cw & ma) && (w & 0xFFFFu) == ra) out |= ma;
+      if ((cw & mb) && (w >> 16) == rb) out |= mb;
+      if (out != cw) M.cw[i] = out;
     }
     __syncthreads();
   }

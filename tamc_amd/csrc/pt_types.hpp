@@ -137,8 +137,8 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align((kPhaseSlots + 1u) * 8u);
   L.fmask = o;  // ICM flip masks [kMaxPairs / 4][16] (groups with one plane word per site)
   o += lds_align(P > 0 ? kMaxPairs / 4u * 16u * 8u : 8u);
-  L.fscr = o;  // ICM flip scratch, 64 words per wave
-  o += lds_align(P > 0 ? W * 64u * 4u : 4u);
+  L.fscr = o;  // ICM cluster labelling: a site of each straggling pair's partial cluster
+  o += lds_align(P > 0 ? kMaxPairs * 4u : 4u);
   L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G)
   o += lds_align(W * G * 64u * 8u);
   L.dE = o;  // exchange energy differences [2][B]
