@@ -11,7 +11,8 @@
 //  * slices: every slice launches all resident groups, bucketed by (waves, LDS class) into one
 //    launch per bucket on its own stream so buckets share the CUs.  Group g runs sweeps
 //    [t_done, t_done + S_g) with S_g sized from its measured per-sweep time so that every group
-//    takes about `slice_ms`; requests that arrive meanwhile join at the next slice boundary;
+//    takes about `slice_ms` (stretched to a whole number of sweeps of the slowest group);
+//    requests that arrive meanwhile join at the next slice boundary;
 //  * completion: results of requests that finished in slice k are assembled while slice k+1 runs.
 //
 // Results depend only on (instance, params, seed): the Philox streams are keyed by sweep, site
@@ -342,7 +343,19 @@ struct tamc_server {
     std::vector<std::vector<Bucket *>> per_stream(kStreams);
     uint32_t k = 0;
     for (auto &kv : buckets) per_stream[k++ % kStreams].push_back(&kv.second);
-    const double target_us = opt.slice_ms * 1000.0;
+    // the slice lasts about slice_ms, stretched to a whole number of sweeps of its slowest group
+    // (rounded to nearest) so that group never idles at the slice's end; every other group gets
+    // as many sweeps as fit in that length
+    double target_us = opt.slice_ms * 1000.0;
+    {
+      double longest = 0.0;
+      for (auto &kv : buckets)
+        for (auto &mb : kv.second.members) {
+          const double us = std::max(1.0, mb.first->us_per_sweep[mb.second]);
+          longest = std::max(longest, us * std::max(1.0, std::round(opt.slice_ms * 1000.0 / us)));
+        }
+      target_us = std::max(target_us, longest);
+    }
     *slice_ms = 0.0;
     bool launched[kStreams] = {false, false, false};
     std::vector<std::vector<std::pair<Bucket *, uint32_t>>> offsets(kStreams);
@@ -354,7 +367,7 @@ struct tamc_server {
           Request *r = b->members[m].first;
           const uint32_t gi = b->members[m].second;
           const uint32_t left = r->S.T - r->t_done[gi];
-          const double want = std::floor(target_us / std::max(1.0, r->us_per_sweep[gi]));
+          const double want = std::floor(target_us / std::max(1.0, r->us_per_sweep[gi]) + 1e-9);
           const uint32_t sweeps = (uint32_t)std::min<double>(left, std::max(1.0, want));
           h_groups[s][base + m] = r->S.groups[gi].g;
           h_ranges[s][2 * (base + m)] = r->t_done[gi];
