@@ -26,6 +26,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # aggregate LDS bandwidth assumed for the LDS-level fraction: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
+# kernel phases timed per group (tamc_batch_stats.phase_ms order)
+PHASES = ["icm_overlap", "icm_seed", "icm_growth", "icm_flip", "exchange", "sweep", "measure"]
 BYTES_PER_ATTEMPT_D3 = 9 * 3 + 9  # 4 B col + 4 B J + 1 B spin per edge; 4 B row_ptr + 4 B bias + 1 B spin per site
 
 
@@ -183,7 +185,7 @@ def other_configs(with_cpu):
     shapes = {"2d_L128_pmJ": W.lattice_2d(128, np.random.default_rng(8), pm_j=True),
               "3d_L24_gauss": W.lattice_3d(24, np.random.default_rng(7)),
               "rr3_N16384_gauss": W.random_regular3(16384, np.random.default_rng(5), gaussian=True)}
-    names = ["icm_overlap", "icm_seed", "icm_growth", "icm_flip", "exchange", "sweep", "measure"]
+    names = PHASES
     largest = {}
     for name, (i, j, K, n) in shapes.items():
         inst = api.BqmIsingInstance.from_lines(i, j, K)
@@ -313,6 +315,9 @@ def main():
         "host_ms_per_step": {"prep": float(np.mean(prep_ms)), "post": float(np.mean(post_ms)),
                              "other": ms_per_step - float(np.mean(dev_ms)) - float(np.mean(prep_ms))
                              - float(np.mean(post_ms))},
+        # the launch lasts as long as its slowest group (one instance per CU): that group's phases
+        "slowest_group_us_per_sweep": {k2: v * 1e3 / args.sweeps for k2, v in zip(PHASES, st["phase_ms"])},
+        "slowest_group_icm_passes_per_sweep": st["icm_passes"] / args.sweeps,
         "plain_gpu": {"value": plain_rate, "unit": "spin-flip attempts/s", "instances": k,
                       "note": "one instance per kernel launch, launched one after another"},
         "batched_vs_plain": (value / plain_rate) if plain_rate else None,
