@@ -226,6 +226,18 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// OR of a predicate over the work group with one barrier (__syncthreads_or takes two).  Round k
+// of a loop writes slot k % 3 and clears slot (k + 2) % 3, which round k - 1 read before round k's
+// barrier and round k + 2 writes after round k + 1's; slots 0 and 1 start at zero.
+__device__ __forceinline__ bool block_any(uint32_t tid, uint32_t lane, uint32_t *slots, uint32_t k, bool v) {
+  const uint64_t b = __builtin_amdgcn_ballot_w64(v);
+  if (lane == 0u && b != 0ull) slots[k % 3u] = 1u;
+  __syncthreads();
+  const bool any = slots[k % 3u] != 0u;
+  if (tid == 0u) slots[(k + 2u) % 3u] = 0u;
+  return any;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
   v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
@@ -434,7 +446,9 @@ __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, u
                      : reinterpret_cast<const TAMC_AS4 uint64_t *>(M.nbr_glb)[RW * i + k];
   };
   uint32_t *keep = c.fscr;  // [p]: a site of pair p's partial cluster
+  uint32_t *any_slots = c.misc + 12, rnd = 0u;
   if (c.tid < 32u) keep[c.tid] = 0xFFFFFFFFu;
+  if (c.tid < 2u) any_slots[c.tid] = 0u;
   __syncthreads();
   for (uint32_t i = c.tid; i < n; i += c.nthreads) {
     const uint32_t cw = M.cw[i], q = M.qw[i];
@@ -493,7 +507,7 @@ __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, u
           }
         }
       }
-      if (!__syncthreads_or(hooked)) break;
+      if (!block_any(c.tid, c.lane, any_slots, rnd++, hooked != 0)) break;
       for (uint32_t i = c.tid; i < n; i += c.nthreads) {
         const uint32_t w = M.qw[i];
         if (w == 0xFFFFFFFFu) continue;
@@ -681,7 +695,10 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   } else if (D > 4u && D <= 8u && n <= 16u * c.nthreads) {
     icm_grow_registers<16, 2>(c, M, n, D);
   } else {
-    for (;;) {
+    uint32_t *any_slots = c.misc + 12;
+    if (c.tid < 2u) any_slots[c.tid] = 0u;
+    __syncthreads();
+    for (uint32_t rnd = 0;; ++rnd) {
       if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
       int changed = 0;
       for (uint32_t rep = 0; rep < 2u; ++rep)
@@ -716,7 +733,7 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
           changed = 1;
         }
       }
-      if (!__syncthreads_or(changed)) break;
+      if (!block_any(c.tid, c.lane, any_slots, rnd, changed != 0)) break;
     }
   }
 }
