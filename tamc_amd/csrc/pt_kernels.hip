@@ -1087,7 +1087,10 @@ template <int D>
 __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t G = c.G;
-  long long acc[4] = {0, 0, 0, 0};
+  // accepted dE in fixed point, per (wave, plane word, lane) slot of the LDS partials: only this
+  // lane adds into its slot, and no accumulator array is carried in registers through the loop
+  unsigned long long *acc = reinterpret_cast<unsigned long long *>(c.partial) + c.wave * G * 64u + c.lane;
+  for (uint32_t g = 0; g < G; ++g) acc[g * 64u] = 0ull;
   const uint32_t half = c.lane >> 5, sh = c.lane & 31u;
   const uint32_t nl = X.n_levels;
   const bool self = X.self_J != nullptr;
@@ -1186,21 +1189,11 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         }
       }
       // the group's accepted dE, summed in group order, enter the fixed-point total
-      const long long fx = to_fix(a_g);
-      if (G == 1u) {
-        acc[0] += fx;
-      } else {
-#pragma unroll
-        for (uint32_t gg = 0; gg < 4u; ++gg)
-          if (gg == g) acc[gg] += fx;
-      }
+      atomicAdd(acc + g * 64u, (unsigned long long)to_fix(a_g));
     }
     a = b;
     __syncthreads();
   }
-#pragma unroll
-  for (uint32_t g = 0; g < 4u; ++g)
-    if (g < G) c.partial[(c.wave * G + g) * 64u + c.lane] = acc[g];
   __syncthreads();
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
     const uint32_t g = r >> 6, l = r & 63u;
