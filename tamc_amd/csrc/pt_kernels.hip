@@ -1125,8 +1125,9 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
           uint32_t w[D];
 #pragma unroll
           for (int k = 0; k < D; ++k) w[k] = c.sp[(rec[8u + s * D + k] * G + g) * 2u + half];
-          float f = 0.0f;
-          f += __uint_as_float(rec[4u + s]);
+          // the reference starts from 0.0 + bias; starting from the bias differs only in the sign
+          // of a zero field, which changes no decision (dE = +-0 always accepts) and no energy
+          float f = __uint_as_float(rec[4u + s]);
 #pragma unroll
           for (int k = 0; k < D; ++k)
             f = __fmaf_rn(__uint_as_float(rec[8u + 4u * D + s * D + k]),
@@ -1149,10 +1150,12 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
           }
         }
       }
+      // decisions are kept as wave masks (one compare per ballot, combined on the scalar unit);
+      // inverse_ballot turns a mask back into the lane's predicate at no vector cost
+      const uint64_t vmask = ballot(valid);
       float dd[4], xs[4];
       uint32_t ks[4];
-      bool ok[4];
-      uint64_t band[4], any_band = 0ull;
+      uint64_t okm[4], band[4], any_band = 0ull;
 #pragma unroll
       for (uint32_t s = 0; s < 4u; ++s) {
         dd[s] = d[s] * (bi[s] ? 2.0f : -2.0f);  // dE *= -2*s_i
@@ -1162,26 +1165,31 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         // products round exactly as accept_below(P) and reject_above(P) do
         const float p = __builtin_amdgcn_exp2f(xs[s] * 1.44269504088896341f);
         const float kf = (float)ks[s];
-        const bool neg = dd[s] < 0.0f;
-        const bool lo = kf < __fmaf_rn(p, 16777216.0f * (1.0f - kBand), -0x1p-100f);
-        const bool hi = kf > p * (16777216.0f * (1.0f + kBand));
-        ok[s] = neg || lo;
-        band[s] = ballot(!neg && !lo && !hi && valid && site[s] != kNone);
+        const uint64_t acc_m = ballot(dd[s] < 0.0f) | ballot(kf < __fmaf_rn(p, 16777216.0f * (1.0f - kBand), -0x1p-100f));
+        const uint64_t rej_m = ballot(kf > p * (16777216.0f * (1.0f + kBand)));
+        okm[s] = acc_m;
+        band[s] = (site[s] != kNone) ? ~(acc_m | rej_m) & vmask : 0ull;
         any_band |= band[s];
       }
       if (any_band != 0ull) {
 #pragma unroll
-        for (uint32_t s = 0; s < 4u; ++s)
-          if ((band[s] >> c.lane) & 1ull) ok[s] = accept_exact(xs[s], ks[s], exc_table());
+        for (uint32_t s = 0; s < 4u; ++s) {
+          if (band[s] == 0ull) continue;
+          bool e = false;
+          if (__builtin_amdgcn_inverse_ballot_w64(band[s])) e = accept_exact(xs[s], ks[s], exc_table());
+          okm[s] |= ballot(e);
+        }
       }
       float a_g = 0.f;
 #pragma unroll
       for (uint32_t s = 0; s < 4u; ++s) {
         if (site[s] == kNone) break;
         const uint32_t i = site[s];
-        const bool acc_ok = ok[s] && valid;
-        const uint64_t nw = ballot((bi[s] ^ (acc_ok ? 1u : 0u)) != 0u);
+        const uint64_t accm = okm[s] & vmask;
+        // the site's new plane word: the lanes' old bits with the accepted ones flipped
+        const uint64_t nw = ballot(bi[s] != 0u) ^ accm;
         if (c.lane == 0u) *reinterpret_cast<uint64_t *>(c.sp + (i * G + g) * 2u) = nw;
+        const bool acc_ok = __builtin_amdgcn_inverse_ballot_w64(accm);
         if (acc_ok) a_g += dd[s];
         if (self) {
           const float sj = c4(X.self_J)[i];
