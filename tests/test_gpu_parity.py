@@ -160,9 +160,11 @@ def test_pt_production_large_instances(gpu, kind, kw, exact):
 
 def test_icm_growth_every_sites_per_thread(gpu):
     # one wave per work group (64 threads): the cluster growth keeps 4, 8 or 16 sites per thread in
-    # registers, for neighbour rows of one 64-bit word (width 4) and of two (3D lattices, width 6)
+    # registers, for neighbour rows of one 64-bit word (width 4) and of two (3D lattices, width 6);
+    # beyond 16 sites per thread it loops over memory
     cases = [("2d_pm", dict(L=12)), ("rr3_pm", dict(n=400)), ("2d_pm", dict(L=30)),
-             ("3d_pm", dict(L=6)), ("3d_pm", dict(L=7)), ("3d_pm", dict(L=10))]
+             ("3d_pm", dict(L=6)), ("3d_pm", dict(L=7)), ("3d_pm", dict(L=10)),
+             ("2d_pm", dict(L=34)), ("3d_pm", dict(L=11))]
     B, C, T = 8, 2, 12
     insts, params, oracles = [], [], []
     for k, (kind, kw) in enumerate(cases):
@@ -172,7 +174,7 @@ def test_icm_growth_every_sites_per_thread(gpu):
         params.append(api.PtIcmParams(num_sweeps=T, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, B),
                                       lo_num_beta=4, icm=True, num_replica_chains=C, threads=1, sample=2,
                                       sample_states=3, sample_limiting=2, seed=900 + k))
-    assert [g.n for g in insts] == [144, 400, 900, 216, 343, 1000]
+    assert [g.n for g in insts] == [144, 400, 900, 216, 343, 1000, 1156, 1331]
     batch, st = api.run_pt_batch(insts, params, waves_per_group=1)
     assert st["waves_per_group"] == 1
     for k, (oi, p) in enumerate(zip(oracles, params)):
@@ -181,6 +183,23 @@ def test_icm_growth_every_sites_per_thread(gpu):
                      threaded=False, sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
         assert o["error"] == 0
         assert_same(batch[k], o, exact_energy=True)
+
+
+def test_icm_staged_growth_over_memory(gpu):
+    # 2D L=100 with 32 betas x 2 chains: the ICM words do not fit in LDS beside the spins, so the
+    # growth runs in the spin planes' LDS while the planes are parked; at 8 waves (512 threads)
+    # 10000 sites exceed 16 per thread, so it loops over memory there
+    oi, gi, _ = make_pair("2d_pm", seed=61, L=100)
+    p = api.PtIcmParams(num_sweeps=6, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 32),
+                        lo_num_beta=16, icm=True, num_replica_chains=2, threads=1, sample=2, sample_states=3,
+                        sample_limiting=2, seed=4242)
+    batch, st = api.run_pt_batch([gi], [p], waves_per_group=8)
+    assert st["waves_per_group"] == 8
+    runner = api.PtIcmRunner(gi, p)
+    o = O.pt_run(oi, runner.betas, 6, num_chains=2, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=True,
+                 threaded=False, sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
+    assert o["error"] == 0
+    assert_same(batch[0], o, exact_energy=True)
 
 
 def test_pt_large_ladder_two_words(gpu):
