@@ -97,11 +97,13 @@ def serving_leg(n_req, rate, sweeps, slice_ms):
                 time.sleep(delay)
             tickets.append(srv.submit(insts[k], params(k)))
         submitted_s = time.perf_counter() - t0
-        lat, attempts = [], 0
-        for t in tickets:
+        lat, attempts, info = [], 0, []
+        for k, t in enumerate(tickets):
             res, tm = srv.wait(t)
             lat.append(tm["total_ms"])
             attempts += res["attempts"]
+            info.append({"request": k, "kind": reqs[k][1], "size": reqs[k][2], "arrival_s": round(reqs[k][0], 4),
+                         "total_ms": tm["total_ms"], "queued_ms": tm["queued_ms"], "slices": tm["slices"]})
         makespan = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array(lat)
@@ -111,7 +113,10 @@ def serving_leg(n_req, rate, sweeps, slice_ms):
         "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
         "throughput_attempts_per_s": attempts / makespan, "makespan_s": makespan, "arrivals_s": submitted_s,
         "offered_attempts_per_s": attempts / reqs[-1][0], "slices": st["slices"],
-        "device_busy_s": st["device_busy_ms"] / 1e3,
+        "device_busy_s": st["device_busy_ms"] / 1e3, "slice_wall_s": st["slice_wall_ms"] / 1e3,
+        "complete_s": st["complete_ms"] / 1e3,
+        # the slowest requests: shape, arrival, wait for admission and slices taken
+        "slowest": sorted(info, key=lambda x: -x["total_ms"])[:4],
     }
 
 

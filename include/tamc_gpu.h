@@ -236,6 +236,8 @@ typedef struct {
   double device_busy_ms;     /* sum of slice device times */
   uint64_t attempts;         /* Metropolis attempts of completed requests */
   uint64_t pool_used, pool_bytes;
+  double slice_wall_ms;      /* sum of slice wall-clock times (launch to the last stream's end) */
+  double complete_ms;        /* host time downloading and assembling finished requests */
 } tamc_server_stats;
 
 tamc_status tamc_server_create(const tamc_server_options *opts, tamc_server **out);

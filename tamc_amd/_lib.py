@@ -93,7 +93,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("cancelled", C.c_uint64), ("queued", C.c_uint32), ("resident", C.c_uint32),
                 ("resident_waves", C.c_uint32), ("slices", C.c_uint64), ("device_busy_ms", C.c_double),
-                ("attempts", C.c_uint64), ("pool_used", C.c_uint64), ("pool_bytes", C.c_uint64)]
+                ("attempts", C.c_uint64), ("pool_used", C.c_uint64), ("pool_bytes", C.c_uint64),
+                ("slice_wall_ms", C.c_double), ("complete_ms", C.c_double)]
 
 
 class Partition(C.Structure):

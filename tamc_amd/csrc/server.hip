@@ -9,10 +9,11 @@
 //  * residency: an admitted request's spins, ladder and outputs stay in its pool block until it
 //    finishes; nothing is copied back in between;
 //  * slices: every slice launches all resident groups, bucketed by (waves, LDS class) into one
-//    launch per bucket on its own stream so buckets share the CUs.  Group g runs sweeps
-//    [t_done, t_done + S_g) with S_g sized from its measured per-sweep time so that every group
-//    takes about `slice_ms` (stretched to a whole number of sweeps of the slowest group);
-//    requests that arrive meanwhile join at the next slice boundary;
+//    launch per bucket.  The bucket holding the slowest group runs alone on one stream; the
+//    others share the remaining streams, one after another on each, so buckets share the CUs.
+//    Group g runs sweeps [t_done, t_done + S_g) with S_g sized from its measured per-sweep time
+//    so that every stream takes about `slice_ms` (stretched to a whole number of sweeps of the
+//    slowest group); requests that arrive meanwhile join at the next slice boundary;
 //  * completion: results of requests that finished in slice k are assembled while slice k+1 runs.
 //
 // Results depend only on (instance, params, seed): the Philox streams are keyed by sweep, site
@@ -134,6 +135,7 @@ struct tamc_server {
   const uint32_t **h_init_ptrs = nullptr;
   const uint32_t **d_init_ptrs = nullptr;
   hipStream_t streams[kStreams] = {};
+  hipStream_t copy_stream = nullptr;  // result downloads, beside the slices
   hipEvent_t ev_init = nullptr, ev_start[kStreams] = {}, ev_end[kStreams] = {};
 
   std::mutex mu;
@@ -166,6 +168,7 @@ struct tamc_server {
     if (d_init_groups) (void)hipFree(d_init_groups);
     if (d_init_ptrs) (void)hipFree((void *)d_init_ptrs);
     if (ev_init) (void)hipEventDestroy(ev_init);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
 
   tamc_status setup() {
@@ -178,8 +181,16 @@ struct tamc_server {
     if (opt.waves_per_group > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 0..16");
     max_groups = std::max(1u, opt.max_wave_slots);  // at least one wave per group
     TAMC_HIP(pool.init(opt.pool_bytes));
+    // stream 0 carries the slice's slowest groups and may use every CU; the other streams leave
+    // the first eighth of the CUs free, so that those groups never wait for a whole CU to drain of
+    // smaller work groups
+    std::vector<uint32_t> mask((cus + 31u) / 32u, 0u);
+    for (uint32_t cu = cus / 8u; cu < cus; ++cu) mask[cu / 32u] |= 1u << (cu % 32u);
     for (uint32_t k = 0; k < kStreams; ++k) {
-      TAMC_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
+      if (k == 0u || cus < 16u)
+        TAMC_HIP(hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking));
+      else
+        TAMC_HIP(hipExtStreamCreateWithCUMask(&streams[k], (uint32_t)mask.size(), mask.data()));
       TAMC_HIP(hipEventCreate(&ev_start[k]));
       TAMC_HIP(hipEventCreate(&ev_end[k]));
       TAMC_HIP(hipHostMalloc((void **)&h_groups[k], max_groups * sizeof(PtGroup), hipHostMallocDefault));
@@ -189,6 +200,7 @@ struct tamc_server {
       TAMC_HIP(hipMalloc((void **)&d_ranges[k], max_groups * 2 * sizeof(uint32_t)));
       TAMC_HIP(hipMalloc((void **)&d_ticks[k], max_groups * sizeof(unsigned long long)));
     }
+    TAMC_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     TAMC_HIP(hipEventCreate(&ev_init));
     TAMC_HIP(hipHostMalloc((void **)&h_init_groups, max_groups * sizeof(PtGroup), hipHostMallocDefault));
     TAMC_HIP(hipHostMalloc((void **)&h_init_ptrs, max_groups * sizeof(void *), hipHostMallocDefault));
@@ -330,44 +342,55 @@ struct tamc_server {
 
   // One slice over the resident groups; returns the slice's device time in ms.
   tamc_status run_slice(double *slice_ms) {
+    // buckets by (waves, LDS class); 16-wave groups fill a CU whatever their LDS, so they share
+    // one bucket
     std::map<std::pair<uint32_t, uint32_t>, Bucket> buckets;
     for (Request *r : resident)
       for (uint32_t gi = 0; gi < r->S.groups.size(); ++gi) {
         if (r->t_done[gi] >= r->S.T) continue;
-        Bucket &b = buckets[{r->waves, r->lds_class}];
+        Bucket &b = buckets[{r->waves, r->waves >= 16u ? 0u : r->lds_class}];
         b.waves = r->waves;
         b.lds = std::max(b.lds, r->lds);
         b.members.push_back({r, gi});
       }
-    // buckets are spread over the streams; each stream's buckets run one after another
+    // The slice lasts about slice_ms, stretched to a whole number of sweeps (rounded to nearest)
+    // of its slowest group so that group never idles at the slice's end.  That group's bucket
+    // runs alone on stream 0; the other buckets are spread over the other streams, where they run
+    // one after another and share the slice's length equally.
+    Bucket *crit = nullptr;
+    double target_us = opt.slice_ms * 1000.0, slowest = 0.0;
+    for (auto &kv : buckets)
+      for (auto &mb : kv.second.members) {
+        const double us = std::max(1.0, mb.first->us_per_sweep[mb.second]);
+        if (us > slowest) {
+          slowest = us;
+          crit = &kv.second;
+        }
+      }
+    if (crit) target_us = std::max(target_us, slowest * std::max(1.0, std::round(opt.slice_ms * 1000.0 / slowest)));
     std::vector<std::vector<Bucket *>> per_stream(kStreams);
     uint32_t k = 0;
-    for (auto &kv : buckets) per_stream[k++ % kStreams].push_back(&kv.second);
-    // the slice lasts about slice_ms, stretched to a whole number of sweeps of its slowest group
-    // (rounded to nearest) so that group never idles at the slice's end; every other group gets
-    // as many sweeps as fit in that length
-    double target_us = opt.slice_ms * 1000.0;
-    {
-      double longest = 0.0;
-      for (auto &kv : buckets)
-        for (auto &mb : kv.second.members) {
-          const double us = std::max(1.0, mb.first->us_per_sweep[mb.second]);
-          longest = std::max(longest, us * std::max(1.0, std::round(opt.slice_ms * 1000.0 / us)));
-        }
-      target_us = std::max(target_us, longest);
+    for (auto &kv : buckets) {
+      if (&kv.second == crit) {
+        per_stream[0].push_back(crit);
+      } else {
+        per_stream[1u + k % (kStreams - 1u)].push_back(&kv.second);
+        ++k;
+      }
     }
     *slice_ms = 0.0;
     bool launched[kStreams] = {false, false, false};
     std::vector<std::vector<std::pair<Bucket *, uint32_t>>> offsets(kStreams);
     for (uint32_t s = 0; s < kStreams; ++s) {
       uint32_t base = 0;
+      const double budget_us = per_stream[s].empty() ? 0.0 : target_us / (double)per_stream[s].size();
       for (Bucket *b : per_stream[s]) {
         offsets[s].push_back({b, base});
         for (size_t m = 0; m < b->members.size(); ++m) {
           Request *r = b->members[m].first;
           const uint32_t gi = b->members[m].second;
           const uint32_t left = r->S.T - r->t_done[gi];
-          const double want = std::floor(target_us / std::max(1.0, r->us_per_sweep[gi]) + 1e-9);
+          const double want = std::floor(budget_us / std::max(1.0, r->us_per_sweep[gi]) + 1e-9);
           const uint32_t sweeps = (uint32_t)std::min<double>(left, std::max(1.0, want));
           h_groups[s][base + m] = r->S.groups[gi].g;
           h_ranges[s][2 * (base + m)] = r->t_done[gi];
@@ -428,18 +451,22 @@ struct tamc_server {
   }
 
   void complete(Request *r) {
+    const Clock::time_point t0 = Clock::now();
     tamc_pt_result *res = nullptr;
     // one bulk download of the request's pool block, then assembly from the host copy
     std::vector<char> host(r->pool_sz);
     const char *dev = (const char *)pool.base + r->pool_off;
     tamc_status s = TAMC_OK;
-    if (hipMemcpy(host.data(), dev, r->pool_sz, hipMemcpyDeviceToHost) != hipSuccess) {
+    // on a non-blocking stream of its own: the slice streams may be blocking (CU-masked) ones
+    if (hipMemcpyAsync(host.data(), dev, r->pool_sz, hipMemcpyDeviceToHost, copy_stream) != hipSuccess ||
+        hipStreamSynchronize(copy_stream) != hipSuccess) {
       s = TAMC_E_DEVICE;
     } else {
       const HostMirror m{host.data(), dev, r->pool_sz};
       s = assemble(r->S, &res, &m);
     }
     std::lock_guard<std::mutex> lk(mu);
+    st.complete_ms += ms_between(t0, Clock::now());
     if (s) {
       fail_request(r, s, "result download failed");
       return;
@@ -489,7 +516,9 @@ struct tamc_server {
       }
       tamc_status s = init_fresh(fresh);
       double slice_ms = 0.0;
+      const Clock::time_point t_slice = Clock::now();
       if (!s && !resident.empty()) s = run_slice(&slice_ms);
+      const double wall_ms = ms_between(t_slice, Clock::now());
       if (s) {
         // a device error ends every resident request with it
         std::lock_guard<std::mutex> lk(mu);
@@ -510,6 +539,7 @@ struct tamc_server {
       std::lock_guard<std::mutex> lk(mu);
       st.slices++;
       st.device_busy_ms += slice_ms;
+      st.slice_wall_ms += wall_ms;
       for (Request *r : std::vector<Request *>(resident)) {
         r->slices++;
         bool all_done = true;
