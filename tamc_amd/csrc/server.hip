@@ -425,12 +425,18 @@ struct tamc_server {
     // while the slice runs: assemble the requests that finished in the previous slice
     for (Request *r : finished_prev) complete(r);
     finished_prev.clear();
+    for (uint32_t s = 0; s < kStreams; ++s)
+      if (launched[s]) TAMC_HIP(hipStreamSynchronize(streams[s]));
     for (uint32_t s = 0; s < kStreams; ++s) {
       if (!launched[s]) continue;
-      TAMC_HIP(hipStreamSynchronize(streams[s]));
       float ms = 0.f;
-      TAMC_HIP(hipEventElapsedTime(&ms, ev_start[s], ev_end[s]));
-      *slice_ms = std::max(*slice_ms, (double)ms);
+      // the slice's device time: from the earliest stream start to the latest stream end (the
+      // streams overlap, so this is the largest start-to-end distance over all stream pairs)
+      for (uint32_t a = 0; a < kStreams; ++a) {
+        if (!launched[a]) continue;
+        TAMC_HIP(hipEventElapsedTime(&ms, ev_start[a], ev_end[s]));
+        *slice_ms = std::max(*slice_ms, (double)ms);
+      }
       for (auto &bo : offsets[s]) {
         Bucket *b = bo.first;
         for (size_t m = 0; m < b->members.size(); ++m) {
