@@ -97,6 +97,7 @@ struct Request {
   std::vector<uint32_t *> init_dev;  // per group, device copy of the caller's initial states
   size_t pool_off = 0, pool_sz = 0;
   bool in_pool = false, initialized = false;
+  bool init_pending = false;  // initialised on the init stream, joins the slices once that is done
   uint32_t waves = 8, lds = 0, lds_class = 0;
   std::vector<uint32_t> t_done;
   std::vector<double> us_per_sweep;
@@ -135,6 +136,9 @@ struct tamc_server {
   const uint32_t **h_init_ptrs = nullptr;
   const uint32_t **d_init_ptrs = nullptr;
   hipStream_t streams[kStreams] = {};
+  hipStream_t init_stream = nullptr;  // uploads and initial states of admitted requests
+  bool init_outstanding = false;      // ev_init marks init work not yet waited for
+  std::vector<Request *> initializing;
   hipStream_t copy_stream = nullptr;  // result downloads, beside the slices
   hipEvent_t ev_init = nullptr, ev_start[kStreams] = {}, ev_end[kStreams] = {};
 
@@ -168,6 +172,7 @@ struct tamc_server {
     if (d_init_groups) (void)hipFree(d_init_groups);
     if (d_init_ptrs) (void)hipFree((void *)d_init_ptrs);
     if (ev_init) (void)hipEventDestroy(ev_init);
+    if (init_stream) (void)hipStreamDestroy(init_stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
   }
 
@@ -202,6 +207,7 @@ struct tamc_server {
     }
     TAMC_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     TAMC_HIP(hipEventCreate(&ev_init));
+    TAMC_HIP(hipStreamCreateWithFlags(&init_stream, hipStreamNonBlocking));
     TAMC_HIP(hipHostMalloc((void **)&h_init_groups, max_groups * sizeof(PtGroup), hipHostMallocDefault));
     TAMC_HIP(hipHostMalloc((void **)&h_init_ptrs, max_groups * sizeof(void *), hipHostMallocDefault));
     TAMC_HIP(hipMalloc((void **)&d_init_groups, max_groups * sizeof(PtGroup)));
@@ -310,17 +316,32 @@ struct tamc_server {
   }
 
   // Device work for the requests admitted since the last slice: initial states and energies.
+  // Uploads and initial states of the requests admitted at this boundary run on the init stream,
+  // beside the slice that follows; those requests join the slices after it.  The host waits for
+  // the previous boundary's init work (normally long done) before reusing its staging buffers
+  // and releasing its requests into the slices.
+  // waits for the outstanding init work; its requests may then join the slices (or be released)
+  tamc_status drain_init() {
+    if (!init_outstanding) return TAMC_OK;
+    init_outstanding = false;
+    for (Request *r : initializing) r->init_pending = false;
+    initializing.clear();
+    TAMC_HIP(hipEventSynchronize(ev_init));
+    return TAMC_OK;
+  }
+
   tamc_status init_fresh(std::vector<Request *> &fresh) {
+    if (tamc_status s = drain_init()) return s;
     uint32_t ng = 0;
     bool any_init = false;
     for (Request *r : fresh) {
-      tamc_status s = fill_groups(r->S, streams[0]);
+      tamc_status s = fill_groups(r->S, init_stream);
       if (s) return s;
       for (size_t gi = 0; gi < r->S.groups.size(); ++gi) {
         GroupPlan &g = r->S.groups[gi];
         if (!g.init_bits.empty()) {
           TAMC_HIP(hipMemcpyAsync(r->init_dev[gi], g.init_bits.data(), g.init_bits.size() * 4,
-                                  hipMemcpyHostToDevice, streams[0]));
+                                  hipMemcpyHostToDevice, init_stream));
           any_init = true;
         }
         h_init_groups[ng] = g.g;
@@ -328,15 +349,17 @@ struct tamc_server {
         ++ng;
       }
       r->initialized = true;
+      r->init_pending = true;
+      initializing.push_back(r);
     }
     if (!ng) return TAMC_OK;
-    TAMC_HIP(hipMemcpyAsync(d_init_groups, h_init_groups, ng * sizeof(PtGroup), hipMemcpyHostToDevice, streams[0]));
+    TAMC_HIP(hipMemcpyAsync(d_init_groups, h_init_groups, ng * sizeof(PtGroup), hipMemcpyHostToDevice, init_stream));
     if (any_init)
       TAMC_HIP(hipMemcpyAsync((void *)d_init_ptrs, (const void *)h_init_ptrs, ng * sizeof(void *),
-                              hipMemcpyHostToDevice, streams[0]));
-    TAMC_HIP(tamc_launch_pt_init(d_init_groups, ng, any_init ? d_init_ptrs : nullptr, streams[0]));
-    TAMC_HIP(hipEventRecord(ev_init, streams[0]));
-    for (uint32_t k = 1; k < kStreams; ++k) TAMC_HIP(hipStreamWaitEvent(streams[k], ev_init, 0));
+                              hipMemcpyHostToDevice, init_stream));
+    TAMC_HIP(tamc_launch_pt_init(d_init_groups, ng, any_init ? d_init_ptrs : nullptr, init_stream));
+    TAMC_HIP(hipEventRecord(ev_init, init_stream));
+    init_outstanding = true;
     return TAMC_OK;
   }
 
@@ -347,7 +370,7 @@ struct tamc_server {
     std::map<std::pair<uint32_t, uint32_t>, Bucket> buckets;
     for (Request *r : resident)
       for (uint32_t gi = 0; gi < r->S.groups.size(); ++gi) {
-        if (r->t_done[gi] >= r->S.T) continue;
+        if (r->init_pending || r->t_done[gi] >= r->S.T) continue;
         Bucket &b = buckets[{r->waves, r->waves >= 16u ? 0u : r->lds_class}];
         b.waves = r->waves;
         b.lds = std::max(b.lds, r->lds);
@@ -413,8 +436,10 @@ struct tamc_server {
                               streams[s]));
       launched[s] = true;
     }
+    slice_launched = launched[0] || launched[1] || launched[2];
     return collect_after_launch(per_stream, offsets, launched, slice_ms);
   }
+  bool slice_launched = false;
 
   std::vector<Request *> finished_prev;
 
@@ -498,7 +523,10 @@ struct tamc_server {
       std::vector<Request *> fresh;
       {
         std::unique_lock<std::mutex> lk(mu);
-        // cancelled requests leave the queue, and the device at this boundary
+        // cancelled requests leave the queue, and the device at this boundary (after any init
+        // work still writing their state)
+        for (Request *r : resident)
+          if (r->cancel && r->init_pending) (void)drain_init();
         for (Request *r : std::vector<Request *>(resident))
           if (r->cancel) {
             leave_resident(r);
@@ -527,6 +555,7 @@ struct tamc_server {
       const double wall_ms = ms_between(t_slice, Clock::now());
       if (s) {
         // a device error ends every resident request with it
+        (void)drain_init();
         std::lock_guard<std::mutex> lk(mu);
         const std::string m = tamc_last_error();
         for (Request *r : std::vector<Request *>(resident)) {
@@ -543,10 +572,12 @@ struct tamc_server {
         continue;
       }
       std::lock_guard<std::mutex> lk(mu);
+      if (!slice_launched) continue;  // every resident request was still being initialised
       st.slices++;
       st.device_busy_ms += slice_ms;
       st.slice_wall_ms += wall_ms;
       for (Request *r : std::vector<Request *>(resident)) {
+        if (r->init_pending) continue;
         r->slices++;
         bool all_done = true;
         for (uint32_t gi = 0; gi < r->S.groups.size(); ++gi) all_done &= r->t_done[gi] >= r->S.T;
