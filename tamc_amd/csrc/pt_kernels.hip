@@ -424,7 +424,7 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
 //    label; a shortcut round then points every label at its root.  When a hook round finds
 //    nothing to do, labels are constant on components, and the cluster is every member site with
 //    the label of the kept site.
-constexpr uint32_t kLabelAfterPasses = 12u, kMaxLabelPairs = 16u, kNoLabel = 0xFFFFu;
+constexpr uint32_t kLabelAfterPasses = 10u, kMaxLabelPairs = 16u, kNoLabel = 0xFFFFu;
 
 // lowers the 16-bit half `h` (0 low, 1 high) of an LDS word to at most l
 template <class U32P>

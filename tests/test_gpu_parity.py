@@ -81,7 +81,7 @@ def test_houdayer_long_clusters(gpu, long_pairs):
     # q<0 on a serpentine through a periodic 32x32 lattice: the even rows but the last, joined
     # through one site of each odd row, alternately at columns 0 and 16, so that the cluster is
     # 527 sites whose far end is 150 to 270 hops from the seed.  With at most 16 such pairs among
-    # 32 the clusters are finished by component labelling after the first 24 passes; with 20 the
+    # 32 the clusters are finished by component labelling after the first 10 passes; with 20 the
     # bit-parallel growth goes on until no more than 16 are still growing.  Both must give the
     # reference's flip sets.
     L = 32
