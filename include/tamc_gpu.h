@@ -48,7 +48,32 @@ tamc_status tamc_instance_from_file(const char *path, int qubo, tamc_instance **
 /* Same semantics from in-memory lines (i[k], j[k], K[k]) in file order. */
 tamc_status tamc_instance_from_lines(uint64_t nlines, const uint32_t *i, const uint32_t *j, const double *K,
                                      int qubo, tamc_instance **out);
-/* BqmIsingInstance::with_suscept (src/ising.rs:388-403): nchi vectors of n coefficients (row-major). */
+/* The fields of BqmIsingInstance (src/ising.rs:257-263) from caller arrays, for callers that already
+ * hold one (from new_zero_bias, src/ising.rs:266-285, from_instance_file, or built by hand):
+ * coupling_vecs as CSR -- row i is entries row_ptr[i] .. row_ptr[i+1]-1 in the row's own order,
+ * which is the order the fields are summed in (QUBO self entries (i, K_ii/2) included, as
+ * src/ising.rs:308-310 pushes them) -- plus bias, offset and suscept_coefs.  The coupling CsMat
+ * (the ICM graph, src/pt.rs:343-346) is the off-diagonal pattern of coupling_vecs, as both
+ * reference constructors build it.  That pattern must be symmetric (j in row i <=> i in row j for
+ * i != j): a Houdayer cluster is a connected component of an undirected graph.  The arrays are
+ * copied. */
+typedef struct {
+  uint32_t n;
+  uint32_t nnz;
+  const uint32_t *row_ptr;  /* [n+1]: row_ptr[0] = 0, non-decreasing, row_ptr[n] = nnz */
+  const uint32_t *col;      /* [nnz], each < n */
+  const float *J;           /* [nnz] */
+  const float *bias;        /* [n]; null = zero biases (new_zero_bias) */
+  float offset;
+  uint32_t nchi;
+  const double *suscept;    /* [nchi][n] suscept_coefs; null when nchi = 0 */
+  int32_t qubo;             /* reported by tamc_instance_get_info; changes nothing else */
+} tamc_instance_desc;
+tamc_status tamc_instance_create(const tamc_instance_desc *desc, tamc_instance **out);
+/* BqmIsingInstance::with_suscept (src/ising.rs:388-403): nchi vectors of n coefficients (row-major).
+ * Like the reference's builder, this is for set-up: an instance must not change while a call that
+ * uses it runs, and both suscept setters fail with TAMC_E_INVALID_ARG while server requests on the
+ * instance are queued or resident. */
 tamc_status tamc_instance_set_suscept(tamc_instance *inst, uint32_t nchi, const double *w);
 /* read_txt_vec (src/util.rs:112-139) on one file, appended as one more suscept vector.  Like the
  * reference, a length mismatch is reported on stdout and the vector kept; runs then fail. */
@@ -169,6 +194,27 @@ tamc_status tamc_pt_run_lockstep(const tamc_instance *inst, const tamc_pt_params
                                  const uint64_t *stream_lens, tamc_pt_result **out);
 void tamc_pt_result_free(tamc_pt_result *r);
 
+/* Vec<PTState> (tamc-core/src/pt.rs:18-25) as a run's starting point, for PtIcmRunner::run(Some(states))
+ * (src/pt.rs:397-408), which continues the given states, counters and round-trip tags.  Every array
+ * is in slot order ([chain][beta slot]); the final_* fields of a tamc_pt_result are exactly these
+ * arrays, so one run's result resumes in the next.  A null counter array means the value
+ * PTState::new gives (tamc-core/src/pt.rs:27-38): zeros, and tags MinBeta at slot 0, MaxBeta at
+ * slot B-1, None elsewhere. */
+typedef struct {
+  const int8_t *states;            /* [C][B][n] spins (+1/-1): PTState.states[b].arr */
+  const float *energies;           /* [C][B] cached IsingState energies; null = energy_ref of each state */
+  const uint32_t *num_acceptances; /* [C][B] */
+  const uint32_t *round_trips;     /* [C] */
+  const uint8_t *tags;             /* [C][B] PTRoundTrip ordinals (0..4, as tamc_pt_result.tags) */
+  const uint32_t *diffusion_hist;  /* [C][B][2] */
+} tamc_pt_state;
+
+/* PtIcmRunner::run(Some(states)) (src/pt.rs:397-408): tamc_pt_run from a full PTState.  acceptance
+ * counts in the result include the imported num_acceptances, as count_acc (src/pt.rs:585-591)
+ * sums the carried-over counters. */
+tamc_status tamc_pt_run_resume(const tamc_instance *inst, const tamc_pt_params *params, const tamc_pt_state *state,
+                               tamc_pt_result **out);
+
 /* ------------------------------------------------------------ batched PT ---- */
 typedef struct {
   uint32_t waves_per_group;  /* 0 = chosen by the autotuner when `autotune` is set, else from the
@@ -213,7 +259,9 @@ tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts,
  * on the device and advance in time slices that all resident requests share, so requests that
  * arrive while others run join at the next slice.  Each result is identical to tamc_pt_run on
  * the same (instance, params, seed).  All functions are thread-safe; the instance passed to
- * tamc_server_submit must stay alive until the request has finished or been cancelled. */
+ * tamc_server_submit must stay alive until the request has finished or been cancelled, and it is
+ * immutable meanwhile: tamc_instance_set_suscept and tamc_instance_add_suscept_file fail with
+ * TAMC_E_INVALID_ARG while a request on it is queued or resident. */
 typedef struct tamc_server tamc_server;
 
 typedef struct {

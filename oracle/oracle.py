@@ -30,7 +30,8 @@ class PtParams(C.Structure):
         ("sample_states_enabled", C.c_int32), ("sample_states", C.c_uint32), ("sample_limiting", C.c_uint32),
         ("rng_mode", C.c_int32), ("seed", C.c_uint64), ("num_streams", C.c_uint32),
         ("stream_words", C.POINTER(_u32p)), ("stream_lens", _u64p), ("init_states", _i8p),
-        ("sweep_order", C.c_uint32),
+        ("sweep_order", C.c_uint32), ("init_energies", _f32p), ("init_nacc", _u32p), ("init_rt", _u32p),
+        ("init_tags", _u8p), ("init_hist", _u32p), ("energy_fixed", C.c_uint32),
     ]
 
 
@@ -291,8 +292,12 @@ def _arr(ptr, count, dtype):
 
 def pt_run(inst, betas, num_sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_idx=0, icm=True, threaded=False,
            sample=None, sample_states=None, sample_limiting=0, rng_mode=RNG_PHILOX, seed=0, streams=None,
-           init_states=None, sweep_order="colour"):
-    """Run PT-ICM with the restated loop; returns a dict of numpy arrays (slot order)."""
+           init_states=None, sweep_order="colour", init_state=None, energy_fixed=None):
+    """Run PT-ICM with the restated loop; returns a dict of numpy arrays (slot order).
+    init_state: optional dict with the rest of a PTState to resume from (energies, num_acceptances,
+    round_trips, tags, diffusion_hist; each optional), in slot order.
+    energy_fixed: the production path's exact tracked-energy accounting (default: on in Philox mode,
+    which restates the production path; the reference's f32 accumulation otherwise)."""
     betas = np.ascontiguousarray(betas, np.float32)
     B = len(betas)
     P = PtParams()
@@ -312,6 +317,7 @@ def pt_run(inst, betas, num_sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_i
     P.rng_mode = rng_mode
     P.seed = seed
     P.sweep_order = {"colour": 0, "sequential": 1}[sweep_order]
+    P.energy_fixed = int(rng_mode == RNG_PHILOX if energy_fixed is None else energy_fixed)
     keep = []
     if streams is not None:
         arrs = [np.ascontiguousarray(s, np.uint32) for s in streams]
@@ -326,6 +332,14 @@ def pt_run(inst, betas, num_sweeps, num_chains=2, warmup_fraction=0.5, lo_beta_i
         init = np.ascontiguousarray(init_states, np.int8)
         keep.append(init)
         P.init_states = _p(init, _i8p)
+    for key, field, dt, pt in [("energies", "init_energies", np.float32, _f32p),
+                               ("num_acceptances", "init_nacc", np.uint32, _u32p),
+                               ("round_trips", "init_rt", np.uint32, _u32p), ("tags", "init_tags", np.uint8, _u8p),
+                               ("diffusion_hist", "init_hist", np.uint32, _u32p)]:
+        if init_state is not None and init_state.get(key) is not None:
+            a = np.ascontiguousarray(init_state[key], dt)
+            keep.append(a)
+            setattr(P, field, _p(a, pt))
     rp = lib.tamco_pt_run(inst.h, C.byref(P))
     R = rp.contents
     try:

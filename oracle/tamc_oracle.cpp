@@ -309,6 +309,51 @@ float sweep(const Inst &I, int8_t *s, float beta, U &&draw, const uint32_t *orde
   return de;
 }
 
+// Tracked-energy accounting of the production path (not a reference concept; selected by
+// energy_fixed): energy changes are summed exactly as integers in units of 2^-32, and the sum is
+// added to the f32 energy once per sweep (or per ICM move).
+inline long long to_fix(float x) { return (long long)std::rint((double)x * 4294967296.0); }
+inline float add_fix(float e, long long s) { return e + (float)((double)s * 2.3283064365386962890625e-10); }
+
+// The Metropolis sweep of `sweep` with that accounting.  The accepted dE of the (up to) four sites
+// of one RNG group are first summed in f32 in the group's site order, starting from 0; the group
+// sums enter the exact total.  Returns that total; `drift` gains -2 J_ii for every accepted flip of
+// a site with a QUBO self entry J_ii (those flips move the tracked energy by dE_true - 2 J_ii,
+// src/ising.rs:308-310 and 468-480).
+template <class U>
+long long sweep_fixed(const Inst &I, int8_t *s, float beta, U &&draw, const uint32_t *order, const uint32_t *gid,
+                      std::vector<float> &group_sum, const float *self_J, long long *drift) {
+  std::fill(group_sum.begin(), group_sum.end(), 0.0f);
+  for (uint32_t k = 0; k < I.n; ++k) {
+    const uint32_t i = order ? order[k] : k;
+    const float d = delta_energy(I, s, i);
+    bool ok = d < 0.0f;
+    if (!ok) ok = draw(i) < expf(-beta * d);
+    if (ok) {
+      s[i] = (int8_t)-s[i];
+      group_sum[gid[i]] += d;
+      if (self_J[i] != 0.0f) *drift += to_fix(-2.0f * self_J[i]);
+    }
+  }
+  long long acc = 0;
+  for (float x : group_sum) acc += to_fix(x);
+  return acc;
+}
+
+// The exact energy change of replica `a` when the cluster's spins flip: the sum over cluster sites
+// i of -2 s_i (h_i + sum of J_ij s_j over the row entries j outside the cluster, in row order).
+long long cluster_delta_fixed(const Inst &I, const int8_t *a, const std::vector<uint8_t> &cl) {
+  long long tot = 0;
+  for (uint32_t i = 0; i < I.n; ++i) {
+    if (!cl[i]) continue;
+    float hh = I.bias[i];
+    for (uint32_t e = I.row_ptr[i]; e < I.row_ptr[i + 1]; ++e)
+      if (!cl[I.col[e]]) hh += I.J[e] * (float)a[I.col[e]];
+    tot += to_fix(a[i] < 0 ? 2.0f * hh : -2.0f * hh);
+  }
+  return tot;
+}
+
 // Production sweep schedule of the GPU path (not a reference concept).  Sites are split into
 // classes of mutually uncoupled sites and swept class by class, ascending inside a class:
 //   order 0 (colour): greedy colouring in site order, colour(i) = the smallest colour not taken by
@@ -437,6 +482,7 @@ struct St {
   float energy = 0.f;
   bool init = false;
   uint32_t id = 0;
+  long long drift = 0;  // energy_fixed: tracked energy minus energy_ref, from QUBO self entries
 };
 
 struct Chain {  // tamc-core/src/pt.rs:18-48
@@ -525,6 +571,9 @@ struct PtRun {
   std::vector<uint64_t> gs_states;
   uint64_t icm_moves = 0, attempts = 0;
   RngGroups groups;
+  bool fixed = false;          // production tracked-energy accounting (energy_fixed)
+  std::vector<float> self_J;   // QUBO self entry of each site (0 where none)
+  std::vector<float> group_sum;  // per RNG group f32 sums of a sweep's accepted dE
 
   PtRun(const Inst &I_, const tamco_pt_params &P_) : I(I_), P(P_), groups(I_, P_.sweep_order) {
     n = I.n;
@@ -543,6 +592,12 @@ struct PtRun {
     chi.assign(B, std::vector<std::vector<float>>(nchi));
     const uint32_t lvl = P.sample_limiting;
     samples.assign(lvl == 0 ? B : (lvl == 1 ? B / 2 : 1), {});
+    fixed = P.rng_mode == TAMCO_RNG_PHILOX && P.energy_fixed;
+    group_sum.assign((size_t)n + 4, 0.0f);  // at most one RNG group per site
+    self_J.assign(n, 0.0f);
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint32_t e = I.row_ptr[i]; e < I.row_ptr[i + 1]; ++e)
+        if (I.col[e] == i) self_J[i] = I.J[e];
   }
 
   uint32_t stream_index(uint32_t c, uint32_t b) const { return P.threaded ? c * B + b : 0; }
@@ -571,7 +626,24 @@ struct PtRun {
           throw std::invalid_argument("feed mode needs init_states");
         }
         energy(I, x);  // rand_ising_state computes the energy (src/ising.rs:175)
+        if (P.init_energies) {  // a resumed IsingState keeps its cached energy
+          const float e = P.init_energies[c * B + b];
+          if (fixed) x.drift = (long long)std::rint(((double)e - (double)x.energy) * 4294967296.0);
+          x.energy = e;
+        }
       }
+    for (uint32_t c = 0; c < C; ++c) {  // the rest of a resumed PTState (tamc-core/src/pt.rs:18-25)
+      Chain &ch = chains[c];
+      for (uint32_t b = 0; b < B; ++b) {
+        if (P.init_nacc) ch.nacc[b] = P.init_nacc[c * B + b];
+        if (P.init_tags) ch.tags[b] = P.init_tags[c * B + b];
+        if (P.init_hist) {
+          ch.hist[2 * b] = P.init_hist[2 * ((size_t)c * B + b)];
+          ch.hist[2 * b + 1] = P.init_hist[2 * ((size_t)c * B + b) + 1];
+        }
+      }
+      if (P.init_rt) ch.rt = P.init_rt[c];
+    }
     // streams
     if (feed) {
       const uint32_t ns = P.threaded ? C * B : 1;
@@ -612,10 +684,21 @@ struct PtRun {
         } else {
           k = streams[stream_index(0, 0)].gen_index(cnt);
         }
-        houdayer_k(I, r1.s.data(), r2.s.data(), k, nullptr);
+        if (fixed) {
+          const std::vector<int8_t> before = r1.s;
+          std::vector<uint8_t> cl;
+          houdayer_k(I, r1.s.data(), r2.s.data(), k, &cl);
+          const long long d = cluster_delta_fixed(I, before.data(), cl);
+          r1.energy = add_fix(r1.energy, d - r1.drift);
+          r2.energy = add_fix(r2.energy, -d - r2.drift);
+          r1.drift = 0;
+          r2.drift = 0;
+        } else {
+          houdayer_k(I, r1.s.data(), r2.s.data(), k, nullptr);
+          r1.init = false;
+          r2.init = false;
+        }
         ++icm_moves;
-        r1.init = false;
-        r2.init = false;
       }
   }
 
@@ -637,7 +720,14 @@ struct PtRun {
     for (uint32_t b = 0; b < B; ++b) {
       St &x = ch.st[b];
       float de;
-      if (P.rng_mode == TAMCO_RNG_PHILOX) {
+      if (fixed) {
+        const long long acc = sweep_fixed(I, x.s.data(), beta[b], PhiloxGroupDraw{P.seed, t, x.id, &groups},
+                                          P.sweep_order == 0 ? groups.order.data() : nullptr, groups.gid.data(),
+                                          group_sum, self_J.data(), &x.drift);
+        x.energy = add_fix(x.energy, acc);
+        attempts += n;
+        continue;
+      } else if (P.rng_mode == TAMCO_RNG_PHILOX) {
         de = sweep(I, x.s.data(), beta[b], PhiloxGroupDraw{P.seed, t, x.id, &groups},
                    P.sweep_order == 0 ? groups.order.data() : nullptr);
       } else {

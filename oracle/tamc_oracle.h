@@ -114,6 +114,19 @@ typedef struct {
   /* TAMCO_RNG_PHILOX only: 0 = sweep colour class by colour class (the GPU's production order),
    * 1 = the reference's sequential order (0..n-1) */
   uint32_t sweep_order;
+  /* resuming from a full PTState (src/pt.rs:397-408), all nullable, slot order: cached energies
+   * [C][B], num_acceptances [C][B], round_trips [C], round-trip tags [C][B], diffusion_hist [C][B][2] */
+  const float *init_energies;
+  const uint32_t *init_nacc;
+  const uint32_t *init_rt;
+  const uint8_t *init_tags;
+  const uint32_t *init_hist;
+  /* TAMCO_RNG_PHILOX only: 1 = the production path's tracked-energy accounting instead of the
+   * reference's f32 accumulation: a sweep's accepted dE are summed exactly (2^-32 fixed point) and
+   * added to the f32 energy once; an ICM move adds the cluster's exact energy change to the tracked
+   * energy (less the QUBO self-entry drift) instead of recomputing energy_ref.  For +-J and QUBO
+   * couplings both give the same bits; for Gaussian couplings they differ in the last bits. */
+  uint32_t energy_fixed;
 } tamco_pt_params;
 
 typedef struct {

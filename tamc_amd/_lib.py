@@ -35,6 +35,17 @@ class InstanceInfo(C.Structure):
                 ("nchi", C.c_uint32), ("offset", C.c_float), ("qubo", C.c_int32)]
 
 
+class InstanceDesc(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("nnz", C.c_uint32), ("row_ptr", _u32p), ("col", _u32p), ("J", _f32p),
+                ("bias", _f32p), ("offset", C.c_float), ("nchi", C.c_uint32), ("suscept", _f64p),
+                ("qubo", C.c_int32)]
+
+
+class PtState(C.Structure):
+    _fields_ = [("states", _i8p), ("energies", _f32p), ("num_acceptances", _u32p), ("round_trips", _u32p),
+                ("tags", _u8p), ("diffusion_hist", _u32p)]
+
+
 class PtParams(C.Structure):
     _fields_ = [("num_sweeps", C.c_uint32), ("warmup_fraction", C.c_double), ("num_betas", C.c_uint32),
                 ("betas", _f32p), ("lo_beta_idx", C.c_uint32), ("icm", C.c_int32),
@@ -116,6 +127,7 @@ SIGNATURES = {
     "tamc_init": (C.c_int, []),
     "tamc_instance_from_file": (C.c_int, [C.c_char_p, C.c_int, _vpp]),
     "tamc_instance_from_lines": (C.c_int, [C.c_uint64, _u32p, _u32p, _f64p, C.c_int, _vpp]),
+    "tamc_instance_create": (C.c_int, [C.POINTER(InstanceDesc), _vpp]),
     "tamc_instance_set_suscept": (C.c_int, [_vp, C.c_uint32, _f64p]),
     "tamc_instance_add_suscept_file": (C.c_int, [_vp, C.c_char_p]),
     "tamc_instance_get_info": (C.c_int, [_vp, C.POINTER(InstanceInfo)]),
@@ -133,6 +145,8 @@ SIGNATURES = {
     "tamc_pt_run": (C.c_int, [_vp, C.POINTER(PtParams), _i8p, C.POINTER(C.POINTER(PtResult))]),
     "tamc_pt_run_lockstep": (C.c_int, [_vp, C.POINTER(PtParams), _i8p, C.c_uint32, C.POINTER(_u32p), _u64p,
                                        C.POINTER(C.POINTER(PtResult))]),
+    "tamc_pt_run_resume": (C.c_int, [_vp, C.POINTER(PtParams), C.POINTER(PtState),
+                                     C.POINTER(C.POINTER(PtResult))]),
     "tamc_pt_result_free": (None, [C.POINTER(PtResult)]),
     "tamc_pt_run_batch": (C.c_int, [C.c_uint32, _vpp, C.POINTER(PtParams), C.POINTER(BatchOptions),
                                     C.POINTER(C.POINTER(PtResult)), C.POINTER(BatchStats)]),

@@ -12,7 +12,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from ._lib import (BatchOptions, BatchStats, InstanceInfo, PtParams as _CPt, PtResult, RequestTimes, SaParams as _CSa,
+from ._lib import (BatchOptions, BatchStats, InstanceDesc, InstanceInfo, PtParams as _CPt, PtResult, PtState,
+                   RequestTimes, SaParams as _CSa,
                    SaResult, ServerOptions, ServerStats, TAMC_E_PENDING, check, lib, ptr, _f32p, _f64p, _i8p, _i64p,
                    _u32p, _u64p, _u8p)
 from . import _lib
@@ -42,6 +43,33 @@ class BqmIsingInstance:
         h = C.c_void_p()
         check(lib().tamc_instance_from_lines(len(i), ptr(i, _u32p), ptr(j, _u32p), ptr(K, _f64p), int(qubo),
                                              C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, row_ptr, col, J, bias=None, offset=0.0, suscept=None, qubo=False):
+        """An instance from the fields of BqmIsingInstance (src/ising.rs:257-263): coupling_vecs as
+        CSR (row order kept, QUBO self entries included), bias (None: zero, as new_zero_bias),
+        offset and suscept_coefs."""
+        rp = np.ascontiguousarray(row_ptr, np.uint32)
+        cl = np.ascontiguousarray(col, np.uint32)
+        Jv = np.ascontiguousarray(J, np.float32)
+        n = len(rp) - 1
+        keep = [rp, cl, Jv]
+        d = InstanceDesc()
+        d.n, d.nnz = n, len(cl)
+        d.row_ptr, d.col, d.J = ptr(rp, _u32p), ptr(cl, _u32p), ptr(Jv, _f32p)
+        if bias is not None:
+            b = np.ascontiguousarray(bias, np.float32)
+            keep.append(b)
+            d.bias = ptr(b, _f32p)
+        d.offset = offset
+        if suscept is not None:
+            w = np.ascontiguousarray(suscept, np.float64).reshape(-1, n)
+            keep.append(w)
+            d.nchi, d.suscept = w.shape[0], ptr(w, _f64p)
+        d.qubo = int(qubo)
+        h = C.c_void_p()
+        check(lib().tamc_instance_create(C.byref(d), C.byref(h)))
         return cls(h.value)
 
     def with_suscept(self, w):
@@ -258,6 +286,30 @@ class PtIcmRunner:
             lib().tamc_pt_result_free(out)
 
     run_parallel = run
+
+    def run_from(self, state):
+        """PtIcmRunner::run(Some(states)) (src/pt.rs:397-408): continue from a full PTState, given
+        as a previous result (its final_states, final_energies, num_acceptances, round_trips, tags
+        and diffusion_hist) or a dict with the keys states, energies, num_acceptances, round_trips,
+        tags, diffusion_hist (all but states optional)."""
+        alias = {"final_states": "states", "final_energies": "energies"}
+        st = {alias.get(k, k): v for k, v in state.items()}
+        P = self._c()
+        S = PtState()
+        keep = []
+        for name, dt, pt in [("states", np.int8, _i8p), ("energies", np.float32, _f32p),
+                             ("num_acceptances", np.uint32, _u32p), ("round_trips", np.uint32, _u32p),
+                             ("tags", np.uint8, _u8p), ("diffusion_hist", np.uint32, _u32p)]:
+            if st.get(name) is not None:
+                a = np.ascontiguousarray(st[name], dt)
+                keep.append(a)
+                setattr(S, name, ptr(a, pt))
+        out = C.POINTER(PtResult)()
+        check(lib().tamc_pt_run_resume(self.instance._h, C.byref(P), C.byref(S), C.byref(out)))
+        try:
+            return pt_result_to_dict(out.contents)
+        finally:
+            lib().tamc_pt_result_free(out)
 
     def run_lockstep(self, init_states, streams):
         """Lock-step mode: every random number from the given u32 word streams (see the C header)."""

@@ -347,10 +347,62 @@ tamc_status tamc_instance_from_lines(uint64_t nlines, const uint32_t *i, const u
   return TAMC_OK;
 }
 
+tamc_status tamc_instance_create(const tamc_instance_desc *d, tamc_instance **out) {
+  clear_error();
+  if (!d || !out) return fail(TAMC_E_INVALID_ARG, "null argument");
+  *out = nullptr;
+  const uint32_t n = d->n;
+  if (!d->row_ptr || (d->nnz && (!d->col || !d->J)) || (d->nchi && !d->suscept))
+    return fail(TAMC_E_INVALID_ARG, "null array in the instance description");
+  if (d->row_ptr[0] != 0u || d->row_ptr[n] != d->nnz) return fail(TAMC_E_INVALID_ARG, "row_ptr must run from 0 to nnz");
+  for (uint32_t i = 0; i < n; ++i)
+    if (d->row_ptr[i + 1] < d->row_ptr[i]) return fail(TAMC_E_INVALID_ARG, "row_ptr must be non-decreasing");
+  for (uint32_t e = 0; e < d->nnz; ++e)
+    if (d->col[e] >= n) return fail(TAMC_E_INVALID_ARG, "column index out of range");
+  // the off-diagonal pattern must be symmetric: count (i, j) and (j, i) occurrences
+  std::vector<std::pair<uint32_t, uint32_t>> fwd, rev;
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t e = d->row_ptr[i]; e < d->row_ptr[i + 1]; ++e)
+      if (d->col[e] != i) {
+        fwd.emplace_back(i, d->col[e]);
+        rev.emplace_back(d->col[e], i);
+      }
+  std::sort(fwd.begin(), fwd.end());
+  std::sort(rev.begin(), rev.end());
+  if (std::adjacent_find(fwd.begin(), fwd.end()) != fwd.end())
+    return fail(TAMC_E_INVALID_ARG, "a row lists the same column twice");
+  if (fwd != rev) return fail(TAMC_E_INVALID_ARG, "the coupling pattern must be symmetric (j in row i <=> i in row j)");
+  tamc_instance *I = new tamc_instance();
+  I->n = n;
+  I->qubo = d->qubo;
+  I->row_ptr.assign(d->row_ptr, d->row_ptr + n + 1);
+  I->col.assign(d->col, d->col + d->nnz);
+  I->J.assign(d->J, d->J + d->nnz);
+  I->bias.assign(n, 0.0f);
+  if (d->bias) I->bias.assign(d->bias, d->bias + n);
+  I->offset = d->offset;
+  // ICM graph: the CsMat of the off-diagonal entries, rows ascending (sprs CSR order)
+  uint32_t gn = 0;
+  for (auto &e : fwd) gn = std::max(gn, std::max(e.first, e.second) + 1u);
+  I->g_nodes = fwd.empty() ? 0u : gn;
+  I->g_row_ptr.assign((size_t)I->g_nodes + 1, 0u);
+  for (auto &e : fwd) I->g_row_ptr[e.first + 1]++;
+  for (uint32_t r = 0; r < I->g_nodes; ++r) I->g_row_ptr[r + 1] += I->g_row_ptr[r];
+  I->g_col.resize(fwd.size());
+  for (size_t k = 0; k < fwd.size(); ++k) I->g_col[k] = fwd[k].second;
+  I->chi.assign(d->nchi, std::vector<double>(n));
+  for (uint32_t k = 0; k < d->nchi; ++k) memcpy(I->chi[k].data(), d->suscept + (size_t)k * n, n * sizeof(double));
+  compute_schedules(I);
+  *out = I;
+  return TAMC_OK;
+}
+
 tamc_status tamc_instance_set_suscept(tamc_instance *inst, uint32_t nchi, const double *w) {
   clear_error();
   if (!inst || (nchi && !w)) return fail(TAMC_E_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> lk(inst->mu);
+  if (inst->server_uses.load())
+    return fail(TAMC_E_INVALID_ARG, "instance has server requests queued or running; it is immutable until they end");
   inst->chi.assign(nchi, std::vector<double>(inst->n));
   for (uint32_t k = 0; k < nchi; ++k) memcpy(inst->chi[k].data(), w + (size_t)k * inst->n, inst->n * sizeof(double));
   if (inst->dev) inst->dev->chi.reset();
@@ -386,6 +438,8 @@ tamc_status tamc_instance_add_suscept_file(tamc_instance *inst, const char *path
     ++ln;
   }
   std::lock_guard<std::mutex> lk(inst->mu);
+  if (inst->server_uses.load())
+    return fail(TAMC_E_INVALID_ARG, "instance has server requests queued or running; it is immutable until they end");
   if (v.size() != inst->n)
     printf("WARNING: Ignoring suscept file %s - Expected %u coefficients, but got %zu\n", path, inst->n, v.size());
   inst->chi.push_back(v);

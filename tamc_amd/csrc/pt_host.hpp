@@ -40,6 +40,7 @@ struct SolvePlan {
   tamc_pt_params P{};
   std::vector<float> betas;
   const int8_t *init_states = nullptr;
+  const tamc_pt_state *resume = nullptr;  // counters, tags and energies to start from (nullable)
   uint32_t n = 0, B = 0, C = 0, T = 0, meas_init = 0, n_meas = 0, n_samp_events = 0, n_samp_slots = 0, nbytes = 0,
            gs_cap = 0, gs_words = 0, npairs = 0, nchi = 0;
   std::vector<GroupPlan> groups;
@@ -49,6 +50,9 @@ struct SolvePlan {
 // Validate one solve and split it into groups (one per ICM chain pair); waves sizes the LDS check.
 tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_params *P, const int8_t *init_states,
                        uint32_t waves);
+// After the init kernel: overwrite the groups' ladder state with the imported PTState fields
+// (tags, acceptance counters, round trips, diffusion histograms, cached energies).
+tamc_status upload_resume(const SolvePlan &S);
 // Dynamic LDS bytes the sweep kernel needs for group g at `waves` waves.
 uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves);
 // Register every device buffer of S in the arena (the ground-state trace in Ags and the betas in

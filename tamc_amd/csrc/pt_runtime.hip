@@ -91,6 +91,34 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
   return TAMC_OK;
 }
 
+tamc_status upload_resume(const SolvePlan &S) {
+  const tamc_pt_state *st = S.resume;
+  if (!st) return TAMC_OK;
+  const uint32_t B = S.B;
+  for (const auto &g : S.groups) {
+    const PtGroup &X = g.g;
+    const size_t c0 = g.chain0, k0 = c0 * B, nk = (size_t)g.nch * B;
+    if (st->tags) TAMC_HIP(hipMemcpy(X.tags, st->tags + k0, nk, hipMemcpyHostToDevice));
+    if (st->num_acceptances) TAMC_HIP(hipMemcpy(X.nacc, st->num_acceptances + k0, nk * 4, hipMemcpyHostToDevice));
+    if (st->diffusion_hist) TAMC_HIP(hipMemcpy(X.hist, st->diffusion_hist + 2 * k0, nk * 8, hipMemcpyHostToDevice));
+    if (st->round_trips) TAMC_HIP(hipMemcpy(X.rt, st->round_trips + c0, (size_t)g.nch * 4, hipMemcpyHostToDevice));
+    // the identity permutation holds after init: physical replica r = local chain * B + slot.
+    // The init kernel left energy_ref of each state in X.energy; a carried energy that differs from
+    // it (QUBO self entries shift the tracked energy, src/ising.rs:308-310) starts as drift, which
+    // the next ICM move on the replica clears as the reference's recomputation does.
+    if (st->energies) {
+      std::vector<float> eref(nk);
+      std::vector<long long> drift(nk);
+      TAMC_HIP(hipMemcpy(eref.data(), X.energy, nk * 4, hipMemcpyDeviceToHost));
+      for (size_t k = 0; k < nk; ++k)
+        drift[k] = (long long)std::rint(((double)st->energies[k0 + k] - (double)eref[k]) * 4294967296.0);
+      TAMC_HIP(hipMemcpy(X.energy, st->energies + k0, nk * 4, hipMemcpyHostToDevice));
+      TAMC_HIP(hipMemcpy(X.drift, drift.data(), nk * 8, hipMemcpyHostToDevice));
+    }
+  }
+  return TAMC_OK;
+}
+
 uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
   return with_nbr(make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global), S.n,
                   g.g.ell_col16 ? g.g.ell_D : 0u, kLdsLimit)
@@ -456,6 +484,8 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   const PtGroup *dg = (const PtGroup *)Lc.d_groups.p;
   TAMC_HIP(tamc_launch_pt_init(dg, ng, Lc.init_ptrs.empty() ? nullptr : (const uint32_t *const *)Lc.d_init_ptrs.p, s));
   TAMC_HIP(hipStreamSynchronize(s));
+  for (const auto &S : solves)
+    if ((st = upload_resume(S))) return st;
   hipEvent_t e0, e1;
   TAMC_HIP(hipEventCreate(&e0));
   TAMC_HIP(hipEventCreate(&e1));
@@ -618,6 +648,23 @@ tamc_status tamc_pt_run(const tamc_instance *inst, const tamc_pt_params *params,
   RunOptions opt;
   opt.waves = inst ? default_waves(inst) : 8;
   if ((st = plan_solve(solves[0], inst, params, init_states, opt.waves))) return st;
+  return run_solves(solves, opt, out, nullptr);
+}
+
+tamc_status tamc_pt_run_resume(const tamc_instance *inst, const tamc_pt_params *params, const tamc_pt_state *state,
+                               tamc_pt_result **out) {
+  clear_error();
+  if (!out || !state || !state->states) return fail(TAMC_E_INVALID_ARG, "null argument");
+  if (params && state->tags)
+    for (size_t k = 0; k < (size_t)params->num_replica_chains * params->num_betas; ++k)
+      if (state->tags[k] > 4u) return fail(TAMC_E_INVALID_ARG, "round-trip tag ordinal out of range (0..4)");
+  tamc_status st = device_init();
+  if (st) return st;
+  std::vector<SolvePlan> solves(1);
+  RunOptions opt;
+  opt.waves = inst ? default_waves(inst) : 8;
+  if ((st = plan_solve(solves[0], inst, params, state->states, opt.waves))) return st;
+  solves[0].resume = state;
   return run_solves(solves, opt, out, nullptr);
 }
 

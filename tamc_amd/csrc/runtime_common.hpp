@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -125,6 +126,9 @@ struct tamc_instance {
   tamc::SiteSchedule sched[2];
   std::mutex mu;
   std::unique_ptr<tamc::DeviceInstance> dev;
+  // server requests on this instance that are queued or resident: they hold device pointers to
+  // its buffers, so the suscept setters refuse to replace them while this is non-zero
+  mutable std::atomic<uint32_t> server_uses{0};
 };
 
 namespace tamc {

@@ -108,6 +108,12 @@ struct Request {
   tamc_pt_result *result = nullptr;
   Clock::time_point t_submit, t_admit, t_ready;
   uint32_t slices = 0;
+  const tamc_instance *inst = nullptr;  // holds one of inst->server_uses until the request ends
+  void drop_instance() {
+    if (inst) inst->server_uses.fetch_sub(1);
+    inst = nullptr;
+  }
+  ~Request() { drop_instance(); }
 };
 
 // Pinned host + device staging for one launch bucket.
@@ -285,6 +291,7 @@ struct tamc_server {
   }
 
   void release_resources(Request *r) {
+    r->drop_instance();
     if (r->in_pool) {
       pool.release(r->pool_off, r->pool_sz);
       r->in_pool = false;
@@ -615,6 +622,11 @@ tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, cons
   if (!srv || !ticket) return fail(TAMC_E_INVALID_ARG, "null argument");
   std::unique_ptr<Request> r(new Request());
   r->t_submit = Clock::now();
+  // taken before planning, so that the suscept vectors the plan reads cannot be replaced under it
+  if (inst) {
+    inst->server_uses.fetch_add(1);
+    r->inst = inst;
+  }
   r->waves = srv->opt.waves_per_group ? srv->opt.waves_per_group : (inst ? default_waves(inst) : 4u);
   tamc_status s = plan_solve(r->S, inst, params, init_states, r->waves);
   if (s) return s;

@@ -157,3 +157,46 @@ def test_thermal_samples_pickle_matches_bincode_fields():
     assert obj["e"] == [[float(x) for x in e[k * 4:(k + 1) * 4]] for k in range(3)]
     assert obj["q"] == [[int(x) for x in q[k * 2:(k + 1) * 2]] for k in range(3)]
     assert obj["suscept"] == [[[float(x) for x in chi[b * 2:(b + 1) * 2]]] for b in range(3)]
+
+
+@pytest.mark.parametrize("kind", ["rr3", "qubo", "2d"])
+def test_instance_create_from_csr_equals_file_build(kind):
+    # tamc_instance_create from the fields of a BqmIsingInstance (src/ising.rs:257-263) gives the same
+    # coupling_vecs, bias, offset and ICM graph as the file build of the same instance
+    rng = np.random.default_rng(12)
+    qubo = kind == "qubo"
+    i, j, K, n = {"rr3": lambda: W.random_regular3(60, rng, bias_sigma=0.3),
+                  "qubo": lambda: W.qubo_sparse(30, 0.2, rng),
+                  "2d": lambda: W.lattice_2d(6, rng)}[kind]()
+    a = api.BqmIsingInstance.from_lines(i, j, K, qubo=qubo)
+    row_ptr, col, J, bias = a.csr()
+    w = rng.standard_normal((2, n))
+    b = api.BqmIsingInstance.from_csr(row_ptr, col, J, bias=bias, offset=a.offset, suscept=w, qubo=qubo)
+    for x, y in zip(a.csr(), b.csr()):
+        assert np.array_equal(x, y)
+    for x, y in zip(a.graph(), b.graph()):
+        assert np.array_equal(x, y)
+    assert (b.n, b.nnz, b.offset, b.qubo, b.graph_nodes) == (a.n, a.nnz, a.offset, a.qubo, a.graph_nodes)
+    info = _lib.InstanceInfo()
+    _lib.check(_lib.lib().tamc_instance_get_info(b._h, C.byref(info)))
+    assert info.nchi == 2
+
+
+def test_instance_create_zero_bias_and_rejections():
+    # new_zero_bias (src/ising.rs:266-285): bias None -> zeros, offset 0
+    row_ptr = np.array([0, 2, 3, 4], np.uint32)
+    col = np.array([1, 2, 0, 0], np.uint32)
+    J = np.array([1.0, -2.0, 1.0, -2.0], np.float32)
+    g = api.BqmIsingInstance.from_csr(row_ptr, col, J)
+    assert np.array_equal(g.csr()[3], np.zeros(3, np.float32)) and g.offset == 0.0
+    bad = [
+        (np.array([0, 2, 3, 3], np.uint32), col[:3], J[:3]),            # (0,2) without (2,0): not symmetric
+        (row_ptr, np.array([1, 2, 0, 5], np.uint32), J),                # column out of range
+        (np.array([0, 3, 2, 4], np.uint32), col, J),                    # row_ptr decreasing
+        (np.array([0, 2, 3, 5], np.uint32), col, J),                    # row_ptr[n] != nnz
+        (np.array([0, 2, 3, 4], np.uint32), np.array([1, 1, 0, 0], np.uint32), J),  # a column twice
+    ]
+    for rp, c, j in bad:
+        with pytest.raises(_lib.TamcError) as e:
+            api.BqmIsingInstance.from_csr(rp, c, j)
+        assert e.value.status == _lib.TAMC_E_INVALID_ARG
