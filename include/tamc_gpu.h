@@ -332,6 +332,19 @@ tamc_status tamc_sa_run_batch(uint32_t count, const tamc_instance *const *insts,
                               tamc_sa_result **results, tamc_batch_stats *stats);
 void tamc_sa_result_free(tamc_sa_result *r);
 
+/* No counterpart in the reference: tamc_sa_run for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * the same kernel and gives the same results as tamc_sa_run, and calls no hipMalloc, hipFree or
+ * other allocating routine: the outputs are caller device buffers, energies [R] floats and states
+ * [R][n] int8 (+1/-1), with R the restarts tamc_sa_run keeps; the schedule's betas are staged in a
+ * static device scratch of 512 KiB per GPU (at most 131072 betas, else TAMC_E_INVALID_ARG), which
+ * every call on that GPU shares and which keeps the last call's contents.  The instance must
+ * already be on the device (any earlier call that ran it uploads it), else TAMC_E_INVALID_ARG.
+ * stream is a hipStream_t (null: the default stream); the work is queued on it and the call returns
+ * without waiting.  Calls that overlap on one GPU race on the scratch. */
+tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params *params, float *energies,
+                                int8_t *states, void *stream);
+
 /* --------------------------------------------------------------- outputs ---- */
 /* bincode-1 serialisation of PtIcmThermalSamples (src/pt.rs:116-125, written at src/pt.rs:652). */
 tamc_status tamc_write_thermal_samples(const tamc_pt_result *r, const float *betas, uint8_t compression_level,

@@ -168,6 +168,7 @@ SIGNATURES = {
     "tamc_sa_run_batch": (C.c_int, [C.c_uint32, _vpp, C.POINTER(SaParams), C.POINTER(C.POINTER(SaResult)),
                                     C.POINTER(BatchStats)]),
     "tamc_sa_result_free": (None, [C.POINTER(SaResult)]),
+    "tamc_sa_run_noalloc": (C.c_int, [_vp, C.POINTER(SaParams), C.c_void_p, C.c_void_p, C.c_void_p]),
     "tamc_write_thermal_samples": (C.c_int, [C.POINTER(PtResult), _f32p, C.c_uint8, C.c_char_p]),
     "tamc_write_anneal_state": (C.c_int, [C.POINTER(SaResult), C.c_char_p]),
     "tamc_write_thermal_samples_pickle": (C.c_int, [C.POINTER(PtResult), _f32p, C.c_uint8, C.c_char_p]),

@@ -300,6 +300,13 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
   return D;
 }
 
+const DeviceInstance *resident_instance(const tamc_instance *cinst) {
+  std::lock_guard<std::mutex> lk(const_cast<tamc_instance *>(cinst)->mu);
+  const DeviceInstance *D = cinst->dev.get();
+  if (!D || !D->row_ptr.p || !D->col.p || !D->J.p || !D->bias.p) return nullptr;
+  return D;
+}
+
 const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st, uint32_t order) {
   DeviceInstance *D = device_instance(inst, G, st, order);
   if (!D) return nullptr;

@@ -137,4 +137,7 @@ namespace tamc {
 DeviceInstance *device_instance(const tamc_instance *inst, uint32_t G, tamc_status *st, uint32_t order = kOrderColour);
 const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st,
                                       uint32_t order = kOrderColour);
+// The instance's device copy (CSR, biases) if an earlier call uploaded it, else nullptr; never
+// allocates.
+const DeviceInstance *resident_instance(const tamc_instance *inst);
 }  // namespace tamc

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "runtime_common.hpp"
@@ -29,8 +31,9 @@ struct SaTask {
   float offset;
   uint32_t n, K, R, rbase;  // rbase: first restart of this wave
   uint64_t seed;
-  uint32_t *spins_out;      // [n][2] bit-plane halves of this wave's restarts
-  float *energy_out;        // [64]
+  uint32_t *spins_out;      // [n][2] bit-plane halves of this wave's restarts (null: states_out)
+  float *energy_out;        // [64]; with states_out: this wave's entries of a [R] array
+  int8_t *states_out;       // null, or this wave's rows of a [R][n] int8 array (+1/-1)
 };
 
 __device__ __forceinline__ uint32_t sa_lane() {
@@ -48,14 +51,9 @@ __device__ __forceinline__ bool sa_accept(float x, uint32_t w) {
   return accept_exact(x, k, ExpfExceptions{kExpfExcX, kExpfExcY, kExpfExcCount});
 }
 
-__global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave) {
-  extern __shared__ __align__(16) uint32_t sa_smem[];
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
-  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
-  if (tix >= ntasks) return;
-  const SaTask &X = tasks[tix];
+// One wave: 64 restarts of one solve, from their initial states to their final energies.
+__device__ __forceinline__ void sa_wave(const SaTask &X, uint32_t *sp) {
   const uint32_t lane = sa_lane(), half = lane >> 5, sh = lane & 31u;
-  uint32_t *sp = sa_smem + (size_t)wave * words_per_wave;
   const uint32_t n = X.n;
   const uint32_t r = X.rbase + lane;
   const bool valid = r < X.R;
@@ -102,9 +100,44 @@ __global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t n
     }
     total = __fadd_rn(total, __fmul_rn(h, si));
   }
+  if (X.states_out) {
+    if (valid) {
+      X.energy_out[lane] = total;
+      int8_t *row = X.states_out + (size_t)lane * n;
+      for (uint32_t i = 0; i < n; ++i) row[i] = ((sp[i * 2u + half] >> sh) & 1u) ? (int8_t)-1 : (int8_t)1;
+    }
+    return;
+  }
   X.energy_out[lane] = total;
   for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
 }
+
+__global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave) {
+  extern __shared__ __align__(16) uint32_t sa_smem[];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
+  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
+  if (tix >= ntasks) return;
+  sa_wave(tasks[tix], sa_smem + (size_t)wave * words_per_wave);
+}
+
+// One solve whose waves are described by one task passed by value: wave w takes restarts
+// 64w .. 64w+63 and writes int8 states and energies straight into the caller's [R] arrays.
+__global__ __launch_bounds__(256) void sa_kernel_direct(SaTask base, uint32_t ntasks, uint32_t words_per_wave) {
+  extern __shared__ __align__(16) uint32_t sa_smem[];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
+  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
+  if (tix >= ntasks) return;
+  SaTask X = base;
+  X.rbase = tix * 64u;
+  X.energy_out = base.energy_out + X.rbase;
+  X.states_out = base.states_out + (size_t)X.rbase * base.n;
+  sa_wave(X, sa_smem + (size_t)wave * words_per_wave);
+}
+
+// Scratch of the allocation-free entry point: the schedule's betas are staged here.  Host code
+// uses the address hipGetSymbolAddress returns for the current device, never the array's name.
+constexpr size_t kNoAllocScratchBytes = 512u * 1024u;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 }  // namespace tamc
 
@@ -232,6 +265,69 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
 }  // namespace
 
 extern "C" {
+
+tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params *params, float *energies,
+                                int8_t *states, void *stream) {
+  clear_error();
+  if (!inst || !params || !params->betas || !energies || !states) return fail(TAMC_E_INVALID_ARG, "null argument");
+  const tamc_sa_params &P = *params;
+  if (P.num_betas < 1) return fail(TAMC_E_INVALID_ARG, "invalid SA parameters");
+  if (P.num_replicas > 1 && P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
+  for (uint32_t k = 1; k < P.num_betas; ++k)
+    if (!(P.betas[k] - P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+  if ((size_t)P.num_betas * 4u > kNoAllocScratchBytes)
+    return fail(TAMC_E_INVALID_ARG, "schedule longer than the allocation-free scratch holds (131072 betas)");
+  const uint32_t n = inst->n;
+  if (n * 8u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20480)");
+  const uint32_t R = P.num_replicas > 1 ? P.threads * (P.num_replicas / P.threads) : P.num_replicas;
+  tamc_status st = device_init();
+  if (st) return st;
+  const DeviceInstance *D = resident_instance(inst);
+  if (!D) return fail(TAMC_E_INVALID_ARG, "instance is not resident on the device: run it once through another entry point");
+  // the scratch address of the current device, looked up once per device
+  static std::mutex mu;
+  static void *dev_scratch[64] = {};
+  int dev = 0;
+  TAMC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(TAMC_E_DEVICE, "device index out of range");
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!dev_scratch[dev]) {
+      void *p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) return fail(TAMC_E_DEVICE, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+      dev_scratch[dev] = p;
+    }
+    base = dev_scratch[dev];
+  }
+  if (R == 0) return TAMC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  float *d_betas = (float *)base;
+  TAMC_HIP(hipMemcpyAsync(d_betas, P.betas, (size_t)P.num_betas * 4u, hipMemcpyHostToDevice, s));
+  SaTask t{};
+  t.row_ptr = (const uint32_t *)D->row_ptr.p;
+  t.col = (const uint32_t *)D->col.p;
+  t.J = (const float *)D->J.p;
+  t.bias = (const float *)D->bias.p;
+  t.betas = d_betas;
+  t.offset = inst->offset;
+  t.n = n;
+  t.K = P.num_betas;
+  t.R = R;
+  t.seed = P.seed;
+  t.energy_out = energies;
+  t.states_out = states;
+  const uint32_t ntasks = (R + 63) / 64, wpw = n * 2u;
+  uint32_t wpg = 4;
+  while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
+  const uint32_t lds = wpg * wpw * 4u;
+  if (lds > 65536u)
+    TAMC_HIP(hipFuncSetAttribute((const void *)sa_kernel_direct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(sa_kernel_direct, dim3((ntasks + wpg - 1) / wpg), dim3(64 * wpg), lds, s, t, ntasks, wpw);
+  TAMC_HIP(hipGetLastError());
+  return TAMC_OK;
+}
 
 tamc_status tamc_sa_run(const tamc_instance *inst, const tamc_sa_params *params, tamc_sa_result **out) {
   return tamc_sa_run_batch(1, &inst, params, out, nullptr);
