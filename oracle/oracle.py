@@ -100,6 +100,7 @@ def _load():
         "tamco_num_negative_overlap": (C.c_uint32, [_i8p, _i8p, C.c_uint32]),
         "tamco_exchange": (None, [C.c_uint32, _f32p, _f32p, _u32p, C.c_int, _u32p, C.c_uint64, _u64p, _u8p, _u32p]),
         "tamco_beta_geometric": (None, [C.c_float, C.c_float, C.c_uint32, _f32p]),
+        "tamco_chain_step": (None, [C.c_uint32, _u8p, _u8p, _u32p, _u32p, _u32p]),
         "tamco_expf": (C.c_float, [C.c_float]),
         "tamco_pt_run": (C.POINTER(PtResult), [C.c_void_p, C.POINTER(PtParams)]),
         "tamco_pt_result_free": (None, [C.POINTER(PtResult)]),
@@ -152,6 +153,24 @@ class Xoshiro:
 
     def clone(self):
         return Xoshiro(state=self.s.copy())
+
+
+class ChainTags:
+    """Round-trip bookkeeping of one PT chain (tags, acceptance counts, diffusion histogram, round
+    trips) driven by given swap decisions; starts as PTState::new (tamc-core/src/pt.rs:27-38)."""
+
+    def __init__(self, B):
+        self.B = B
+        self.tags = np.zeros(B, np.uint8)
+        self.tags[0], self.tags[-1] = 1, 2
+        self.nacc = np.zeros(B, np.uint32)
+        self.hist = np.zeros((B, 2), np.uint32)
+        self.rt = C.c_uint32(0)
+
+    def step(self, swaps):
+        sw = np.ascontiguousarray(swaps, np.uint8)
+        lib.tamco_chain_step(self.B, _p(sw, _u8p), _p(self.tags, _u8p), _p(self.nacc, _u32p), _p(self.hist, _u32p),
+                             C.byref(self.rt))
 
 
 def beta_geometric(bmin, bmax, nb):

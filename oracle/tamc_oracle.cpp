@@ -995,6 +995,22 @@ void tamco_exchange(uint32_t B, const float *betas, const float *energies, uint3
   *cursor = ws.pos;
   if (accept_out && B > 1) std::memcpy(accept_out, acc.data(), B - 1);
 }
+void tamco_chain_step(uint32_t B, const uint8_t *swaps, uint8_t *tags, uint32_t *nacc, uint32_t *hist, uint32_t *rt) {
+  // one exchange pass with the given decisions (swap_states, tamc-core/src/pt.rs:49-53) and the
+  // round-trip update that follows it (tamc-core/src/pt.rs:54-84)
+  Chain ch(B);
+  std::memcpy(ch.tags.data(), tags, B);
+  std::memcpy(ch.nacc.data(), nacc, B * 4u);
+  std::memcpy(ch.hist.data(), hist, 2u * B * 4u);
+  ch.rt = *rt;
+  for (uint32_t j = 0; j + 1 < B; ++j)
+    if (swaps[j]) ch.swap_states(j);
+  ch.update_round_trips();
+  std::memcpy(tags, ch.tags.data(), B);
+  std::memcpy(nacc, ch.nacc.data(), B * 4u);
+  std::memcpy(hist, ch.hist.data(), 2u * B * 4u);
+  *rt = ch.rt;
+}
 void tamco_beta_geometric(float beta_min, float beta_max, uint32_t num_beta, float *out) {
   // tamc-core/src/sa.rs:13-22 in f64, then `as f32` (src/pt.rs:249-250)
   const double l0 = std::log((double)beta_min), lf = std::log((double)beta_max);

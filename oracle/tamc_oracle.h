@@ -84,6 +84,9 @@ uint32_t tamco_num_negative_overlap(const int8_t *s1, const int8_t *s2, uint32_t
 void tamco_exchange(uint32_t B, const float *betas, const float *energies, uint32_t *perm, int variant,
                     const uint32_t *words, uint64_t nwords, uint64_t *cursor, uint8_t *accept_out,
                     uint32_t *nacc);
+/* One exchange pass on one chain with the decisions given (swaps[j] != 0: slots j, j+1 swap, in order
+ * j = 0..B-2), then update_round_trips.  tags [B], nacc [B], hist [B][2] and rt are updated in place. */
+void tamco_chain_step(uint32_t B, const uint8_t *swaps, uint8_t *tags, uint32_t *nacc, uint32_t *hist, uint32_t *rt);
 /* geometric beta schedule cast to f32 (tamc-core/src/sa.rs:13-22 via src/pt.rs:246-256) */
 void tamco_beta_geometric(float beta_min, float beta_max, uint32_t num_beta, float *out);
 /* expf used by the restatement (the reference's f32::exp) */
