@@ -303,7 +303,45 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
 const DeviceInstance *resident_instance(const tamc_instance *cinst) {
   std::lock_guard<std::mutex> lk(const_cast<tamc_instance *>(cinst)->mu);
   const DeviceInstance *D = cinst->dev.get();
-  if (!D || !D->row_ptr.p || !D->col.p || !D->J.p || !D->bias.p) return nullptr;
+  if (!D || !D->sa_info.p || !D->sa_blocks.p) return nullptr;
+  return D;
+}
+
+const DeviceInstance *device_sa_rows(const tamc_instance *cinst, tamc_status *st) {
+  DeviceInstance *D = device_instance(cinst, 1, st);
+  if (!D) return nullptr;
+  tamc_instance *inst = const_cast<tamc_instance *>(cinst);
+  std::lock_guard<std::mutex> lk(inst->mu);
+  if (D->sa_info.p) return D;
+  const uint32_t n = inst->n;
+  std::vector<uint32_t> info((size_t)n * 4), blocks;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t e0 = inst->row_ptr[i], len = inst->row_ptr[i + 1] - e0, nb = (len + 7) / 8;
+    info[4 * i] = (uint32_t)(blocks.size() / 16);
+    info[4 * i + 1] = nb;
+    memcpy(&info[4 * i + 2], &inst->bias[i], 4);
+    info[4 * i + 3] = len;
+    for (uint32_t b = 0; b < nb; ++b) {
+      uint32_t rec[16];
+      for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t e = b * 8 + k;
+        const float J = e < len ? inst->J[e0 + e] : 0.0f;
+        rec[k] = 8u * (e < len ? inst->col[e0 + e] : i);
+        memcpy(&rec[8 + k], &J, 4);
+      }
+      blocks.insert(blocks.end(), rec, rec + 16);
+    }
+  }
+  // the kernel stages rows in chunks of 16 blocks and may read up to 16 blocks past a row's first
+  blocks.resize(blocks.size() + 16u * 16u, 0u);
+  if (D->sa_info.alloc(info.size() * 4) != hipSuccess || D->sa_blocks.alloc(blocks.size() * 4) != hipSuccess ||
+      hipMemcpy(D->sa_info.p, info.data(), info.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(D->sa_blocks.p, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    D->sa_info.reset();
+    D->sa_blocks.reset();
+    *st = fail(TAMC_E_DEVICE, "SA row upload failed");
+    return nullptr;
+  }
   return D;
 }
 

@@ -100,6 +100,10 @@ struct DeviceSchedule {
 
 struct DeviceInstance {
   DevBuf row_ptr, col, J, bias, g_row_ptr, g_col, chi, ell_col, ell_col16, ell_J, self_J;
+  // SA row records: per site {first block, blocks, bias bits, row length}, and blocks of eight
+  // row entries -- eight neighbour byte offsets in a wave's spin planes (8 * col), then the eight
+  // couplings' bits; the last block of a row is padded with the site itself and +0.0
+  DevBuf sa_info, sa_blocks;
   bool has_self = false;
   uint32_t ell_D = 0;
   std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by 2 * (words per site G) + order
@@ -137,7 +141,9 @@ namespace tamc {
 DeviceInstance *device_instance(const tamc_instance *inst, uint32_t G, tamc_status *st, uint32_t order = kOrderColour);
 const DeviceSchedule *device_schedule(const tamc_instance *inst, uint32_t G, tamc_status *st,
                                       uint32_t order = kOrderColour);
-// The instance's device copy (CSR, biases) if an earlier call uploaded it, else nullptr; never
-// allocates.
+// The instance's device copy with its SA row records if an earlier SA call uploaded them, else
+// nullptr; never allocates.
 const DeviceInstance *resident_instance(const tamc_instance *inst);
+// Upload the instance (once) and its SA row records (once); nullptr on error.
+const DeviceInstance *device_sa_rows(const tamc_instance *inst, tamc_status *st);
 }  // namespace tamc

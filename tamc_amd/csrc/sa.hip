@@ -6,6 +6,13 @@
 // The wave's spins are bit planes in LDS (one 64-bit word per site, bit = restart), so a coupling
 // row is fetched once per 64 restarts and each neighbour spin is one broadcast LDS read.  Uniforms
 // come from Philox4x32-10(ctr = {sweep, i>>2, restart, 0}, key = seed), word i&3.
+//
+// A site's row is stored as blocks of eight entries (eight neighbour offsets and eight couplings,
+// 64 bytes).  The next site's row is copied into the wave's LDS with one 16-byte vector load per
+// lane while the current site is processed; the block loop then reads four blocks' entries and
+// their 32 neighbour words together and sums them in row order, so memory latency never sits on
+// the sum's dependency chain.  The last block of a row is padded with the site itself and a +0.0
+// coupling, which adds a signed zero and changes no decision (dE = +-0 accepts either way).
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
@@ -22,11 +29,14 @@
 #define TAMC_EXC_DECL static __device__ const
 #include "generated/expf_exceptions.inc"
 
+#define TAMC_AS3 __attribute__((address_space(3)))
+#define TAMC_AS4 __attribute__((address_space(4)))
+
 namespace tamc {
 
 struct SaTask {
-  const uint32_t *row_ptr, *col;
-  const float *J, *bias;
+  const uint32_t *info;     // [n][4] per site: first block, blocks, bias bits, row length
+  const uint32_t *blocks;   // [blocks][16]: eight neighbour byte offsets (8 * col), eight coupling bits
   const float *betas;
   float offset;
   uint32_t n, K, R, rbase;  // rbase: first restart of this wave
@@ -35,6 +45,11 @@ struct SaTask {
   float *energy_out;        // [64]; with states_out: this wave's entries of a [R] array
   int8_t *states_out;       // null, or this wave's rows of a [R][n] int8 array (+1/-1)
 };
+
+template <class T>
+__device__ __forceinline__ const TAMC_AS4 T *sa_c4(const T *p) {
+  return (const TAMC_AS4 T *)p;
+}
 
 __device__ __forceinline__ uint32_t sa_lane() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -51,52 +66,136 @@ __device__ __forceinline__ bool sa_accept(float x, uint32_t w) {
   return accept_exact(x, k, ExpfExceptions{kExpfExcX, kExpfExcY, kExpfExcCount});
 }
 
+// +1.0 or -1.0 from the spin bit at position 31 - up of w (bit set <=> spin -1)
+__device__ __forceinline__ float sa_sign(uint32_t w, uint32_t up) {
+  return __uint_as_float(0x3f800000u | ((w << up) & 0x80000000u));
+}
+
+// Row staging: a site's row blocks are copied into a per-wave LDS buffer with one 16-byte vector
+// load per lane (16 blocks = 128 entries per chunk); the copy for the next site is loaded while
+// the current site is processed, so the row is read from LDS with no memory latency in the way.
+typedef uint32_t W4 __attribute__((ext_vector_type(4)));  // four words moved as one 16-byte access
+constexpr uint32_t kRowChunkBlocks = 16;
+constexpr uint32_t kRowBufWords = kRowChunkBlocks * 16u;
+// LDS words of one wave: the plane words rounded up to 16 bytes, then the row buffer
+__host__ __device__ __forceinline__ uint32_t sa_plane_words(uint32_t n) { return (2u * n + 3u) & ~3u; }
+__host__ __device__ __forceinline__ uint32_t sa_wave_words(uint32_t n) { return sa_plane_words(n) + kRowBufWords; }
+
 // One wave: 64 restarts of one solve, from their initial states to their final energies.
-__device__ __forceinline__ void sa_wave(const SaTask &X, uint32_t *sp) {
-  const uint32_t lane = sa_lane(), half = lane >> 5, sh = lane & 31u;
-  const uint32_t n = X.n;
+// sp: this wave's LDS, [n][2] plane words followed by the row buffer [kRowBufWords].
+__device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) {
+  const uint32_t lane = sa_lane(), half = lane >> 5, sh = lane & 31u, up = 31u - sh;
+  const uint32_t n = X.n, nq = (n + 3u) / 4u;
   const uint32_t r = X.rbase + lane;
   const bool valid = r < X.R;
   const uint32_t key0 = (uint32_t)X.seed, key1 = (uint32_t)(X.seed >> 32);
-  // initial states: rand_ising_state with the production init stream
-  for (uint32_t i = 0; i < n; ++i) {
-    const U4 o = philox4x32_10(0u, i >> 2, r, kTagInit, key0, key1);
-    const uint32_t bit = valid ? (((pick(o, i) >> 31) ^ 1u) & 1u) : 0u;
-    const uint64_t m = __builtin_amdgcn_ballot_w64(bit != 0u);
+  // this lane's half of every plane word: byte offsets are 8 * site + 4 * half
+  const TAMC_AS3 char *spl = (const TAMC_AS3 char *)sp + half * 4u;
+  TAMC_AS3 W4 *rb = (TAMC_AS3 W4 *)(sp + sa_plane_words(n));
+  const W4 *ginfo = (const W4 *)X.info;
+  const W4 *gblk = (const W4 *)X.blocks;  // block b = W4 4b .. 4b+3
+  auto put = [&](uint32_t i, uint64_t m) {
     if (sh == 0u) sp[i * 2u + half] = half ? (uint32_t)(m >> 32) : (uint32_t)m;
-  }
-  for (uint32_t k = 0; k < X.K; ++k) {
-    const float beta = X.betas[k];
-    U4 u = {0, 0, 0, 0};
-    for (uint32_t i = 0; i < n; ++i) {
-      if ((i & 3u) == 0u) u = philox4x32_10(k, i >> 2, r, kTagSweep, key0, key1);
-      float d = 0.0f;
-      d += X.bias[i];
-      const uint32_t e0 = X.row_ptr[i], e1 = X.row_ptr[i + 1];
-      for (uint32_t e = e0; e < e1; ++e) {
-        const float Je = X.J[e];
-        const uint32_t wj = sp[X.col[e] * 2u + half];
-        d += ((wj >> sh) & 1u) ? -Je : Je;
-      }
-      const uint32_t wi = sp[i * 2u + half];
-      const uint32_t bi = (wi >> sh) & 1u;
-      d *= bi ? 2.0f : -2.0f;
-      bool acc = d < 0.0f;
-      if (!acc) acc = sa_accept(-beta * d, pick(u, i));
-      acc = acc && valid;
-      const uint64_t m = __builtin_amdgcn_ballot_w64((bi ^ (acc ? 1u : 0u)) != 0u);
-      if (sh == 0u) sp[i * 2u + half] = half ? (uint32_t)(m >> 32) : (uint32_t)m;
+  };
+  // initial states: rand_ising_state with the production init stream (one Philox block per quad)
+  for (uint32_t q = 0; q < nq; ++q) {
+    const U4 o = philox4x32_10(0u, q, r, kTagInit, key0, key1);
+    const uint32_t ws[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+    for (uint32_t s = 0; s < 4u; ++s) {
+      const uint32_t i = 4u * q + s;
+      if (i >= n) break;
+      const uint32_t bit = valid ? (((ws[s] >> 31) ^ 1u) & 1u) : 0u;
+      put(i, __builtin_amdgcn_ballot_w64(bit != 0u));
     }
   }
-  // AnnealMinResults::apply_measurements: energy_ref of every final state (src/sa.rs:55-64)
+  // pipeline: site i is processed from rb while the next site's first chunk and the info of the
+  // site after it are in flight
+  W4 inf_c = ginfo[0], inf_n = ginfo[n > 1u ? 1u : 0u];
+  rb[lane] = gblk[(size_t)inf_c.x * 4u + lane];
+  for (uint32_t k = 0; k < X.K; ++k) {
+    const float beta = sa_c4(X.betas)[k];
+    for (uint32_t q = 0; q < nq; ++q) {
+      const U4 u = philox4x32_10(k, q, r, kTagSweep, key0, key1);
+      const uint32_t us[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (uint32_t s = 0; s < 4u; ++s) {
+        const uint32_t i = 4u * q + s;
+        if (i >= n) break;
+        const uint32_t i1 = (i + 1u == n) ? 0u : i + 1u, i2 = (i1 + 1u == n) ? 0u : i1 + 1u;
+        const W4 chunk_n = gblk[(size_t)inf_n.x * 4u + lane];
+        const W4 inf_nn = ginfo[i2];
+        const uint32_t b0 = __builtin_amdgcn_readfirstlane(inf_c.x), nb = __builtin_amdgcn_readfirstlane(inf_c.y);
+        // the reference starts from 0.0 and adds the bias (src/ising.rs:468-480)
+        float d = __fadd_rn(0.0f, __uint_as_float(__builtin_amdgcn_readfirstlane(inf_c.z)));
+        for (uint32_t c0 = 0; c0 < nb; c0 += kRowChunkBlocks) {
+          if (c0 > 0u) rb[lane] = gblk[((size_t)b0 + c0) * 4u + lane];  // rows beyond one chunk
+          const uint32_t cb = (nb - c0 < kRowChunkBlocks) ? nb - c0 : kRowChunkBlocks;
+          // four blocks at a time: their 32 entries, then their 32 neighbour words are read
+          // together, then summed in row order; a remainder of one to three blocks goes singly
+          uint32_t b = 0;
+          for (; b + 4u <= cb; b += 4u) {
+            uint32_t off[32], jb[32], w[32];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+              const W4 o0 = rb[4u * (b + u)], o1 = rb[4u * (b + u) + 1u];
+              const W4 j0 = rb[4u * (b + u) + 2u], j1 = rb[4u * (b + u) + 3u];
+#pragma unroll
+              for (uint32_t e = 0; e < 4u; ++e) {
+                off[8u * u + e] = o0[e];
+                off[8u * u + 4u + e] = o1[e];
+                jb[8u * u + e] = j0[e];
+                jb[8u * u + 4u + e] = j1[e];
+              }
+            }
+#pragma unroll
+            for (int e = 0; e < 32; ++e) w[e] = *(const TAMC_AS3 uint32_t *)(spl + off[e]);
+            // d += J * s_j in row order; J * (+-1.0) is exact, so each fma rounds like the add
+#pragma unroll
+            for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jb[e]), sa_sign(w[e], up), d);
+          }
+          for (; b < cb; ++b) {
+            const W4 o0 = rb[4u * b], o1 = rb[4u * b + 1u], j0 = rb[4u * b + 2u], j1 = rb[4u * b + 3u];
+            uint32_t w[8];
+#pragma unroll
+            for (uint32_t e = 0; e < 4u; ++e) {
+              w[e] = *(const TAMC_AS3 uint32_t *)(spl + o0[e]);
+              w[4u + e] = *(const TAMC_AS3 uint32_t *)(spl + o1[e]);
+            }
+#pragma unroll
+            for (uint32_t e = 0; e < 4u; ++e) d = __fmaf_rn(__uint_as_float(j0[e]), sa_sign(w[e], up), d);
+#pragma unroll
+            for (uint32_t e = 0; e < 4u; ++e) d = __fmaf_rn(__uint_as_float(j1[e]), sa_sign(w[4u + e], up), d);
+          }
+        }
+        const uint32_t bi = (sp[i * 2u + half] >> sh) & 1u;
+        d *= bi ? 2.0f : -2.0f;
+        bool acc = d < 0.0f;
+        if (!acc) acc = sa_accept(-beta * d, us[s]);
+        acc = acc && valid;
+        put(i, __builtin_amdgcn_ballot_w64((bi ^ (acc ? 1u : 0u)) != 0u));
+        rb[lane] = chunk_n;  // after this site's reads of the buffer: LDS operations stay in order
+        inf_c = inf_n;
+        inf_n = inf_nn;
+      }
+    }
+  }
+  // AnnealMinResults::apply_measurements: energy_ref of every final state (src/sa.rs:55-64), the
+  // real row entries in the reference's order
+  const TAMC_AS4 uint32_t *info = sa_c4(X.info);
+  const TAMC_AS4 uint32_t *blocks = sa_c4(X.blocks);
   float total = X.offset;
   for (uint32_t i = 0; i < n; ++i) {
+    const TAMC_AS4 uint32_t *inf = info + 4u * i;
+    const uint32_t b0 = inf[0], len = inf[3];
+    const float si = sa_sign(*(const TAMC_AS3 uint32_t *)(spl + 8u * i), up);
     float h = 0.0f;
-    const float si = ((sp[i * 2u + half] >> sh) & 1u) ? -1.0f : 1.0f;
-    h += X.bias[i];
-    for (uint32_t e = X.row_ptr[i]; e < X.row_ptr[i + 1]; ++e) {
-      const float sj = ((sp[X.col[e] * 2u + half] >> sh) & 1u) ? -1.0f : 1.0f;
-      h += __fmul_rn(X.J[e], sj) / 2.0f;
+    h += __uint_as_float(inf[2]);
+    const TAMC_AS4 uint32_t *blk = blocks + (size_t)b0 * 16u;
+    for (uint32_t e = 0; e < len; ++e) {
+      const uint32_t bl = e >> 3, k = e & 7u;
+      const float sj = sa_sign(*(const TAMC_AS3 uint32_t *)(spl + blk[bl * 16u + k]), up);
+      h += __fmul_rn(__uint_as_float(blk[bl * 16u + 8u + k]), sj) / 2.0f;
     }
     total = __fadd_rn(total, __fmul_rn(h, si));
   }
@@ -112,32 +211,49 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, uint32_t *sp) {
   for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
 }
 
-__global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave) {
+// Work groups are spread over the XCDs round robin (block b runs on XCD b mod 8); the logical
+// group index groups the blocks of one XCD together, so the waves of one solve (consecutive tasks)
+// share that XCD's L2 for their row records.
+__device__ __forceinline__ uint32_t sa_logical_group() {
+  const uint32_t per = (gridDim.x + 7u) / 8u;
+  return (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+}
+
+__global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave,
+                                                 uint32_t ngroups) {
   extern __shared__ __align__(16) uint32_t sa_smem[];
+  const uint32_t g = sa_logical_group();
+  if (g >= ngroups) return;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
-  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
+  const uint32_t tix = g * (blockDim.x / 64u) + wave;
   if (tix >= ntasks) return;
-  sa_wave(tasks[tix], sa_smem + (size_t)wave * words_per_wave);
+  sa_wave(tasks[tix], (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
 }
 
 // One solve whose waves are described by one task passed by value: wave w takes restarts
 // 64w .. 64w+63 and writes int8 states and energies straight into the caller's [R] arrays.
-__global__ __launch_bounds__(256) void sa_kernel_direct(SaTask base, uint32_t ntasks, uint32_t words_per_wave) {
+__global__ __launch_bounds__(256) void sa_kernel_direct(SaTask base, uint32_t ntasks, uint32_t words_per_wave,
+                                                        uint32_t ngroups) {
   extern __shared__ __align__(16) uint32_t sa_smem[];
+  const uint32_t g = sa_logical_group();
+  if (g >= ngroups) return;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
-  const uint32_t tix = blockIdx.x * (blockDim.x / 64u) + wave;
+  const uint32_t tix = g * (blockDim.x / 64u) + wave;
   if (tix >= ntasks) return;
   SaTask X = base;
   X.rbase = tix * 64u;
   X.energy_out = base.energy_out + X.rbase;
   X.states_out = base.states_out + (size_t)X.rbase * base.n;
-  sa_wave(X, sa_smem + (size_t)wave * words_per_wave);
+  sa_wave(X, (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
 }
 
 // Scratch of the allocation-free entry point: the schedule's betas are staged here.  Host code
 // uses the address hipGetSymbolAddress returns for the current device, never the array's name.
 constexpr size_t kNoAllocScratchBytes = 512u * 1024u;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// grid of the launch: 8 * ceil(groups / 8) blocks, so every logical group has a block
+inline uint32_t sa_grid(uint32_t ngroups) { return 8u * ((ngroups + 7u) / 8u); }
 
 }  // namespace tamc
 
@@ -171,25 +287,23 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     A.want(&S.d_spins, (size_t)S.waves * S.inst->n * 2);
     A.want(&S.d_energy, (size_t)S.waves * 64);
   }
-  const uint32_t wpw = maxn * 2u;
+  const uint32_t wpw = sa_wave_words(maxn);  // plane words and the row buffer of one wave
   uint32_t wpg = 4;  // waves per work group
   while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
-  if (wpw * 4u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20480)");
+  if (wpw * 4u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
   SaTask *d_tasks;
   A.want(&d_tasks, ntasks);
   TAMC_HIP(A.commit());
   std::vector<SaTask> tasks;
   for (auto &S : solves) {
     tamc_status st;
-    DeviceInstance *D = device_instance(S.inst, 1, &st);
+    const DeviceInstance *D = device_sa_rows(S.inst, &st);
     if (!D) return st;
     TAMC_HIP(hipMemcpy(S.d_betas, S.P.betas, S.P.num_betas * 4, hipMemcpyHostToDevice));
     for (uint32_t w = 0; w < S.waves; ++w) {
       SaTask t{};
-      t.row_ptr = (const uint32_t *)D->row_ptr.p;
-      t.col = (const uint32_t *)D->col.p;
-      t.J = (const float *)D->J.p;
-      t.bias = (const float *)D->bias.p;
+      t.info = (const uint32_t *)D->sa_info.p;
+      t.blocks = (const uint32_t *)D->sa_blocks.p;
       t.betas = S.d_betas;
       t.offset = S.inst->offset;
       t.n = S.inst->n;
@@ -214,7 +328,9 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
   TAMC_HIP(hipEventCreate(&e1));
   const auto w0 = std::chrono::steady_clock::now();
   TAMC_HIP(hipEventRecord(e0, 0));
-  hipLaunchKernelGGL(sa_kernel, dim3((ntasks + wpg - 1) / wpg), dim3(64 * wpg), wpg * wpw * 4u, 0, d_tasks, ntasks, wpw);
+  const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
+  hipLaunchKernelGGL(sa_kernel, dim3(sa_grid(ngroups)), dim3(64 * wpg), wpg * wpw * 4u, 0, d_tasks, ntasks, wpw,
+                     ngroups);
   TAMC_HIP(hipGetLastError());
   TAMC_HIP(hipEventRecord(e1, 0));
   TAMC_HIP(hipEventSynchronize(e1));
@@ -278,12 +394,13 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   if ((size_t)P.num_betas * 4u > kNoAllocScratchBytes)
     return fail(TAMC_E_INVALID_ARG, "schedule longer than the allocation-free scratch holds (131072 betas)");
   const uint32_t n = inst->n;
-  if (n * 8u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20480)");
+  if (sa_wave_words(n) * 4u > 160u * 1024u)
+    return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
   const uint32_t R = P.num_replicas > 1 ? P.threads * (P.num_replicas / P.threads) : P.num_replicas;
   tamc_status st = device_init();
   if (st) return st;
   const DeviceInstance *D = resident_instance(inst);
-  if (!D) return fail(TAMC_E_INVALID_ARG, "instance is not resident on the device: run it once through another entry point");
+  if (!D) return fail(TAMC_E_INVALID_ARG, "instance is not resident on the device: run SA on it once through tamc_sa_run");
   // the scratch address of the current device, looked up once per device
   static std::mutex mu;
   static void *dev_scratch[64] = {};
@@ -306,10 +423,8 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   float *d_betas = (float *)base;
   TAMC_HIP(hipMemcpyAsync(d_betas, P.betas, (size_t)P.num_betas * 4u, hipMemcpyHostToDevice, s));
   SaTask t{};
-  t.row_ptr = (const uint32_t *)D->row_ptr.p;
-  t.col = (const uint32_t *)D->col.p;
-  t.J = (const float *)D->J.p;
-  t.bias = (const float *)D->bias.p;
+  t.info = (const uint32_t *)D->sa_info.p;
+  t.blocks = (const uint32_t *)D->sa_blocks.p;
   t.betas = d_betas;
   t.offset = inst->offset;
   t.n = n;
@@ -318,13 +433,14 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   t.seed = P.seed;
   t.energy_out = energies;
   t.states_out = states;
-  const uint32_t ntasks = (R + 63) / 64, wpw = n * 2u;
+  const uint32_t ntasks = (R + 63) / 64, wpw = sa_wave_words(n);
   uint32_t wpg = 4;
   while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
   const uint32_t lds = wpg * wpw * 4u;
   if (lds > 65536u)
     TAMC_HIP(hipFuncSetAttribute((const void *)sa_kernel_direct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(sa_kernel_direct, dim3((ntasks + wpg - 1) / wpg), dim3(64 * wpg), lds, s, t, ntasks, wpw);
+  const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
+  hipLaunchKernelGGL(sa_kernel_direct, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, t, ntasks, wpw, ngroups);
   TAMC_HIP(hipGetLastError());
   return TAMC_OK;
 }
