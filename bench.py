@@ -8,8 +8,17 @@ complete batched solve of all 256 instances (T*N*B*C Metropolis attempts summed 
 Instance data is uploaded (and cached on the device) during warm-up, so the timed steps start with
 inputs resident in HBM.
 
-Reported beside it: the plain path (one instance per kernel launch, run one after another) on a
-sample of instances, and the CPU restatement of the reference on the host cores (cpu_baseline).
+Reported beside it:
+  * the plain path (one instance per kernel launch, run one after another) on a sample of instances;
+  * the CPU restatement of the reference on the host cores (cpu_baseline: all the cores this process
+    may run on, and one core);
+  * the other configurations: the 2D L=16 and 3D L=12 single instances on the plain path, SA on 64
+    QUBOs over the full 4096-sweep schedule, each with one-core and all-core CPU figures;
+  * serving: the Poisson request stream at its configured rate (200/s) and at a rate below the
+    server's capacity, p50/p99 time-to-result beside throughput;
+  * for the dominant kernels, the VALU-issue roofline (wave-instructions per attempt from the
+    committed rocprofv3 counter run, times the live attempt rate, against 1 wave-instruction per
+    clock per compute unit) and how busy the mean compute unit is.
 """
 import argparse
 import json
@@ -26,20 +35,47 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # aggregate LDS bandwidth assumed for the LDS-level fraction: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_PEAK_GBS = 256 * 128 * 2.4
+CUS, CLOCK_HZ = 256, 2.4e9
+# VALU issue: one wave64 instruction per clock per compute unit (4 SIMDs, 4 clocks per wave64 op)
+VALU_PEAK = CUS * CLOCK_HZ
 # kernel phases timed per group (tamc_batch_stats.phase_ms order)
 PHASES = ["icm_overlap", "icm_seed", "icm_growth", "icm_flip", "exchange", "sweep", "measure"]
 BYTES_PER_ATTEMPT_D3 = 9 * 3 + 9  # 4 B col + 4 B J + 1 B spin per edge; 4 B row_ptr + 4 B bias + 1 B spin per site
 
 
+def host_cores():
+    """The cores this process may run on (its affinity set), which is what the CPU legs use."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _profile(name):
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def measured_traffic():
     """HBM bytes per launch of pt_sweep_kernel on this workload, from the rocprofv3 FETCH_SIZE and
     WRITE_SIZE runs recorded beside the kernel-trace summary (scaled by the copy calibration)."""
-    path = os.path.join(ROOT, "profiles", "pt_sweep_kernel_rr3x256_traffic.json")
-    try:
-        with open(path) as f:
-            return json.load(f)["traffic_bytes_per_launch"]
-    except (OSError, ValueError, KeyError):
+    t = _profile("pt_sweep_kernel_rr3x256_traffic.json")
+    return t.get("traffic_bytes_per_launch") if t else None
+
+
+def valu_roofline(kernel, attempts_per_s):
+    """VALU-issue roofline from the committed counter run: SQ_INSTS_VALU per attempt of `kernel` on
+    its workload, times the live attempt rate, against VALU_PEAK."""
+    c = _profile("kernel_counters.json")
+    if not c or kernel not in c:
         return None
+    per = c[kernel]["valu_per_attempt"]
+    achieved = per * attempts_per_s
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK, "unit": "wave-instructions/s",
+            "frac": achieved / VALU_PEAK, "valu_per_attempt": per, "counter_workload": c[kernel]["workload"]}
 
 
 def build_workload(count, seed, T):
@@ -74,13 +110,23 @@ def cpu_baseline(lines, params, sweeps, threads):
     return attempts / dt, attempts, dt
 
 
-def serving_leg(n_req, rate, sweeps, slice_ms):
-    """Serving workload: a seeded Poisson stream of mixed PT-ICM requests (2D L 8..128, 3D L 4..24,
-    random 3-regular N 64..16384; +-J or N(0,1); 32 betas 0.2-5.0, 2 chains, `sweeps` sweeps) fed to
-    the continuous-batching server at their arrival times.  Latency = submit -> result on the host."""
-    from tamc_amd import api, workloads
-    reqs = workloads.serving_requests(n_req, rate, seed=105)
-    insts = [api.BqmIsingInstance.from_lines(i, j, K) for (_, _, _, (i, j, K, n)) in reqs]
+def _cpu_parallel(fn, copies, threads):
+    """Run fn(k) for k < copies on `threads` host threads; (attempts/s, attempts, seconds)."""
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        att = sum(ex.map(fn, range(copies)))
+    dt = time.perf_counter() - t0
+    return att / dt, att, dt
+
+
+def serving_leg(reqs, insts, rate, sweeps, slice_ms):
+    """Serving workload: the seeded stream of mixed PT-ICM requests (2D L 8..128, 3D L 4..24, random
+    3-regular N 64..16384; +-J or N(0,1); 32 betas 0.2-5.0, 2 chains, `sweeps` sweeps) fed to the
+    continuous-batching server at arrival times scaled to `rate` requests/s.  Latency = submit ->
+    result on the host."""
+    from tamc_amd import api
+    base_rate = len(reqs) / reqs[-1][0]
+    scale = 200.0 / rate  # the stream was drawn at 200/s; a Poisson process rescales in time
 
     def params(k):
         return api.PtIcmParams(num_sweeps=sweeps, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 32),
@@ -92,27 +138,30 @@ def serving_leg(n_req, rate, sweeps, slice_ms):
         tickets = []
         t0 = time.perf_counter()
         for k, (arrival, _, _, _) in enumerate(reqs):
-            delay = arrival - (time.perf_counter() - t0)
+            delay = arrival * scale - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
             tickets.append(srv.submit(insts[k], params(k)))
         submitted_s = time.perf_counter() - t0
-        lat, attempts, info = [], 0, []
+        lat, queued, attempts, info = [], [], 0, []
         for k, t in enumerate(tickets):
             res, tm = srv.wait(t)
             lat.append(tm["total_ms"])
+            queued.append(tm["queued_ms"])
             attempts += res["attempts"]
-            info.append({"request": k, "kind": reqs[k][1], "size": reqs[k][2], "arrival_s": round(reqs[k][0], 4),
+            info.append({"request": k, "kind": reqs[k][1], "size": reqs[k][2], "arrival_s": round(reqs[k][0] * scale, 4),
                          "total_ms": tm["total_ms"], "queued_ms": tm["queued_ms"], "slices": tm["slices"]})
         makespan = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array(lat)
     return {
-        "requests": n_req, "arrival_rate_per_s": rate, "sweeps": sweeps, "slice_ms": slice_ms,
+        "requests": len(reqs), "arrival_rate_per_s": rate, "realised_rate_per_s": base_rate / scale,
+        "sweeps": sweeps, "slice_ms": slice_ms,
         "p50_ms": float(np.percentile(lat, 50)), "p90_ms": float(np.percentile(lat, 90)),
         "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "queued_p50_ms": float(np.percentile(queued, 50)), "queued_p99_ms": float(np.percentile(queued, 99)),
         "throughput_attempts_per_s": attempts / makespan, "makespan_s": makespan, "arrivals_s": submitted_s,
-        "offered_attempts_per_s": attempts / reqs[-1][0], "slices": st["slices"],
+        "offered_attempts_per_s": attempts / (reqs[-1][0] * scale), "slices": st["slices"],
         "device_busy_s": st["device_busy_ms"] / 1e3, "slice_wall_s": st["slice_wall_ms"] / 1e3,
         "complete_s": st["complete_ms"] / 1e3,
         # the slowest requests: shape, arrival, wait for admission and slices taken
@@ -120,13 +169,24 @@ def serving_leg(n_req, rate, sweeps, slice_ms):
     }
 
 
-def other_configs(with_cpu):
-    """The reference's other configurations, each next to the CPU restatement (one thread, the
-    reference's serial loop with Xoshiro256++) on a bounded sample:
+def other_configs(with_cpu, cores, sa_sweeps):
+    """The reference's other configurations (see pt_plain_cases and sa_case) and the largest
+    serving shapes alone on the device."""
+    out = pt_plain_cases(with_cpu, cores)
+    out["sa_qubo"] = sa_case(with_cpu, cores, sa_sweeps)
+    out["largest_serving_shapes"] = largest_serving_shapes()
+    return out
+
+
+def pt_plain_cases(with_cpu, cores):
+    """The reference's other configurations, each next to the CPU restatement (the reference's
+    serial loop with Xoshiro256++) on a bounded sample, on one core and on all `cores` (that many
+    independent copies of the sample at once: the throughput the host's cores give on the shape):
       plain_2d: 2D L=16 periodic +-J, 32 betas 0.2-5.0, 2 chains, 2000 sweeps (one instance, plain path)
       plain_3d: 3D L=12 periodic N(0,1), 48 betas 0.1-5.0, 4 chains, 10000 sweeps (one instance)
       sa_qubo:  64 QUBOs N=2048, 5% density, Q ~ U{-100..100}, 1024 restarts each, linear beta
-                0.1 -> 10 over 4096 sweeps; timed on the first 64 sweeps of the schedule."""
+                0.1 -> 10 over 4096 sweeps (the full schedule; `sa_sweeps` evenly spaced entries of it
+                when smaller)."""
     from tamc_amd import api, workloads as W
     out = {}
 
@@ -138,59 +198,86 @@ def other_configs(with_cpu):
                             sample_limiting=2, seed=1)
         api.PtIcmRunner(inst, p).run()  # warm (instance upload)
         t0 = time.perf_counter()
-        r = api.PtIcmRunner(inst, p).run()
+        _, st = api.run_pt_batch([inst], [p], plain=True)
         dt = time.perf_counter() - t0
-        rec = {"value": r["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(r["attempts"]),
-               "ms": dt * 1e3, "kernel_ms": r["device_ms"], "n": n}
+        rec = {"value": st["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(st["attempts"]),
+               "ms": dt * 1e3, "kernel_ms": st["device_ms"], "n": n, "work_groups": st["groups"],
+               "mean_cu_busy": st["groups_busy_ms"] / (CUS * st["device_ms"]) if st["device_ms"] else None}
         if with_cpu:
             import oracle as O
             betas, lo = p.resolve()
-            oi = O.Instance.from_lines(i, j, K)
-            t0 = time.perf_counter()
-            o = O.pt_run(oi, betas, cpu_sweeps, num_chains=C, warmup_fraction=0.5, lo_beta_idx=lo, icm=True,
-                         sample=32, sample_states=32, sample_limiting=2, rng_mode=O.RNG_XOSHIRO, seed=1)
-            cdt = time.perf_counter() - t0
-            rec["cpu_1thread"] = {"value": o["attempts"] / cdt, "cores": 1,
-                                  "sample": "%d of %d sweeps" % (cpu_sweeps, T)}
-            rec["vs_cpu_1thread"] = rec["value"] / rec["cpu_1thread"]["value"]
+
+            def one(k):
+                oi = O.Instance.from_lines(i, j, K)
+                return O.pt_run(oi, betas, cpu_sweeps, num_chains=C, warmup_fraction=0.5, lo_beta_idx=lo, icm=True,
+                                sample=32, sample_states=32, sample_limiting=2, rng_mode=O.RNG_XOSHIRO,
+                                seed=1 + k)["attempts"]
+            r1, a1, d1 = _cpu_parallel(one, 1, 1)
+            ra, aa, da = _cpu_parallel(one, cores, cores)
+            rec["cpu_1thread"] = {"value": r1, "cores": 1, "sample": "%d of %d sweeps (%.1f s)" % (cpu_sweeps, T, d1)}
+            rec["cpu_allcore"] = {"value": ra, "cores": cores,
+                                  "sample": "%d independent copies of %d of %d sweeps at once (%.1f s)"
+                                            % (cores, cpu_sweeps, T, da)}
+            rec["vs_cpu_1thread"] = rec["value"] / r1
+            rec["vs_cpu_allcore"] = rec["value"] / ra
         out[name] = rec
 
     pt_case("plain_2d", W.lattice_2d(16, np.random.default_rng(101), pm_j=True), 32, 0.2, 5.0, 2, 2000, 2000)
     pt_case("plain_3d", W.lattice_3d(12, np.random.default_rng(102), pm_j=False), 48, 0.1, 5.0, 4, 10000, 100)
+    return out
 
+
+def sa_case(with_cpu, cores, sa_sweeps):
+    """SA on 64 QUBOs (N=2048, 5 %), 1024 restarts each, the linear 0.1 -> 10 schedule of 4096
+    sweeps (or `sa_sweeps` evenly spaced entries of it), with one-core and all-core CPU figures."""
+    from tamc_amd import api, workloads as W
     insts, lines = [], []
     for k in range(64):
         q = W.qubo_sparse(2048, 0.05, np.random.default_rng(104 + k), qmax=100)
         lines.append(q)
         insts.append(api.BqmIsingInstance.from_lines(q[0], q[1], q[2], qubo=True))
-    S = 64
     betas = (0.1 + 9.9 * np.arange(4096) / 4095).astype(np.float32)
-    ps = [api.SaParams(beta=api.BetaOptions.Arr(betas[:S]), num_replicas=1024, threads=1, seed=k) for k in range(64)]
-    api.run_sa_batch(insts[:1], ps[:1])  # warm
+    sched = betas if sa_sweeps >= 4096 else betas[np.linspace(0, 4095, sa_sweeps).astype(int)]
+    ps = [api.SaParams(beta=api.BetaOptions.Arr(sched), num_replicas=1024, threads=1, seed=k) for k in range(64)]
+    api.run_sa_batch(insts[:1], [api.SaParams(beta=api.BetaOptions.Arr(betas[:4]), num_replicas=64, seed=0)])  # warm
     t0 = time.perf_counter()
     _, st = api.run_sa_batch(insts, ps)
     dt = time.perf_counter() - t0
+    kernel_rate = st["attempts"] / (st["device_ms"] / 1e3)
     rec = {"value": st["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(st["attempts"]),
-           "ms": dt * 1e3, "kernel_ms": st["device_ms"],
-           "sample": "all 64 instances x 1024 restarts, first %d of 4096 sweeps" % S,
-           "roofline_algorithmic_GBs": st["attempts"] / dt * 939 / 1e9}
+           "ms": dt * 1e3, "kernel_ms": st["device_ms"], "kernel_attempts_per_s": kernel_rate,
+           "sample": "all 64 instances x 1024 restarts, %d of the 4096-sweep schedule" % len(sched),
+           # 9 d + 9 algorithmic bytes per attempt at d ~ 103.4 (QUBO self entry included): 939 B
+           "roofline_hbm": {"achieved": kernel_rate * 939 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": kernel_rate * 939 / 1e9 / HBM_PEAK_GBS},
+           "roofline_valu": valu_roofline("sa_kernel", kernel_rate)}
     if with_cpu:
         import oracle as O
-        oi = O.Instance.from_lines(lines[0][0], lines[0][1], lines[0][2], qubo=True)
-        t0 = time.perf_counter()
-        o = O.sa_run(oi, betas[:S], 64, threads=1, rng_mode=O.RNG_XOSHIRO, seed=1)
-        cdt = time.perf_counter() - t0
-        rec["cpu_1thread"] = {"value": o["attempts"] / cdt, "cores": 1, "sample": "1 instance x 64 restarts x %d sweeps" % S}
-        rec["vs_cpu_1thread"] = rec["value"] / rec["cpu_1thread"]["value"]
-    out["sa_qubo"] = rec
+        S = 64
 
+        def one(k):
+            q = lines[k % 64]
+            oi = O.Instance.from_lines(q[0], q[1], q[2], qubo=True)
+            return O.sa_run(oi, betas[:S], 64, threads=1, rng_mode=O.RNG_XOSHIRO, seed=1 + k)["attempts"]
+        r1, a1, d1 = _cpu_parallel(one, 1, 1)
+        ra, aa, da = _cpu_parallel(one, cores, cores)
+        rec["cpu_1thread"] = {"value": r1, "cores": 1, "sample": "1 instance x 64 restarts x %d sweeps (%.1f s)" % (S, d1)}
+        rec["cpu_allcore"] = {"value": ra, "cores": cores,
+                              "sample": "%d instances x 64 restarts x %d sweeps at once (%.1f s)" % (cores, S, da)}
+        rec["vs_cpu_1thread"] = rec["value"] / r1
+        rec["vs_cpu_allcore"] = rec["value"] / ra
+    return rec
+
+
+def largest_serving_shapes():
+    """The largest request shapes of the serving stream, each alone on the device."""
+    from tamc_amd import api, workloads as W
     # the largest request shapes of the serving stream, one instance alone on the device (serving
     # parameters: 32 betas 0.2-5.0, 2 chains, ICM), timed over 100 sweeps: the solve latency floor
     # of the stream's tail, with the kernel's per-phase split
     shapes = {"2d_L128_pmJ": W.lattice_2d(128, np.random.default_rng(8), pm_j=True),
               "3d_L24_gauss": W.lattice_3d(24, np.random.default_rng(7)),
               "rr3_N16384_gauss": W.random_regular3(16384, np.random.default_rng(5), gaussian=True)}
-    names = PHASES
     largest = {}
     for name, (i, j, K, n) in shapes.items():
         inst = api.BqmIsingInstance.from_lines(i, j, K)
@@ -201,10 +288,9 @@ def other_configs(with_cpu):
         api.run_pt_batch([inst], [p], waves_per_group=16)  # warm (instance upload)
         _, st = api.run_pt_batch([inst], [p], waves_per_group=16)
         largest[name] = {"n": n, "us_per_sweep": st["device_ms"] * 1e3 / T,
-                         "phases_us_per_sweep": {k: v * 1e3 / T for k, v in zip(names, st["phase_ms"])},
+                         "phases_us_per_sweep": {k: v * 1e3 / T for k, v in zip(PHASES, st["phase_ms"])},
                          "icm_passes_per_sweep": st["icm_passes"] / T}
-    out["largest_serving_shapes"] = largest
-    return out
+    return largest
 
 
 def main():
@@ -224,12 +310,16 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the other configurations")
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--sa-sweeps", type=int, default=4096, help="SA schedule entries timed (4096 = all)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-low-rate", type=float, default=50.0,
+                    help="second serving rate, below the server's capacity (0 = skip)")
     ap.add_argument("--serve-sweeps", type=int, default=2000)
     ap.add_argument("--serve-slice-ms", type=float, default=4.0)
     ap.add_argument("--only-largest", action="store_true", help="run only the largest instance (profiling)")
     ap.add_argument("--no-icm", action="store_true", help="switch ICM off (profiling the sweep alone)")
+    ap.add_argument("--only-sa", action="store_true", help="run only the SA configuration (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (float32 elementwise), the reference point "
                          "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -251,6 +341,10 @@ def main():
     from tamc_amd import api
     from tamc_amd._lib import check, lib
     check(lib().tamc_init())
+    cores = host_cores()
+    if args.only_sa:
+        print(json.dumps({"sa_qubo": sa_case(False, cores, args.sa_sweeps)}))
+        return 0
     insts, lines, params = build_workload(args.instances, 103, args.sweeps)
     if args.only_largest:
         k = int(np.argmax([x.n for x in insts]))
@@ -267,13 +361,14 @@ def main():
 
     for _ in range(args.warmup):
         step()
-    dev_ms, prep_ms, post_ms = [], [], []
+    dev_ms, prep_ms, post_ms, busy = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res, st = step()
         dev_ms.append(st["device_ms"])
         prep_ms.append(st["prep_ms"])
         post_ms.append(st["post_ms"])
+        busy.append(st["groups_busy_ms"] / (CUS * st["device_ms"]))
         assert st["attempts"] == attempts_per_step
     wall = time.perf_counter() - t0
     value = attempts_per_step * args.steps / wall
@@ -332,17 +427,25 @@ def main():
                      "algorithmic_bytes_per_launch": attempts_per_step * BYTES_PER_ATTEMPT_D3,
                      "kernel_ms": float(np.mean(dev_ms)),
                      # the state lives in LDS for the whole launch: the same bytes against LDS
-                     "lds_level": {"peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": achieved / LDS_PEAK_GBS}},
+                     "lds_level": {"peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": achieved / LDS_PEAK_GBS},
+                     # what does bound it: VALU issue, and the share of the launch the mean CU is busy
+                     "valu": valu_roofline("pt_sweep_kernel", kernel_rate) if default_workload else None,
+                     "mean_cu_busy": float(np.mean(busy))},
     }
     if not args.no_other:
-        out["other_configs"] = other_configs(not args.no_cpu)
+        out["other_configs"] = other_configs(not args.no_cpu, cores, args.sa_sweeps)
     if not args.no_serving:
-        out["serving"] = serving_leg(args.serve_requests, args.serve_rate, args.serve_sweeps, args.serve_slice_ms)
+        from tamc_amd import workloads
+        reqs = workloads.serving_requests(args.serve_requests, 200.0, seed=105)
+        sinsts = [api.BqmIsingInstance.from_lines(i, j, K) for (_, _, _, (i, j, K, n)) in reqs]
+        out["serving"] = serving_leg(reqs, sinsts, args.serve_rate, args.serve_sweeps, args.serve_slice_ms)
+        if args.serve_low_rate > 0:
+            out["serving_below_capacity"] = serving_leg(reqs, sinsts, args.serve_low_rate, args.serve_sweeps,
+                                                        args.serve_slice_ms)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         rate, att, dt = cpu_baseline(lines[:args.cpu_instances], params[:args.cpu_instances], args.cpu_sweeps,
-                                     threads)
-        out["cpu_baseline"] = {"value": rate, "unit": "spin-flip attempts/s", "cores": threads, "kind": "port",
+                                     cores)
+        out["cpu_baseline"] = {"value": rate, "unit": "spin-flip attempts/s", "cores": cores, "kind": "port",
                                "sample": "%d instances x %d sweeps of the same workload (%.3g attempts, %.1f s)"
                                          % (min(args.cpu_instances, len(lines)), args.cpu_sweeps, att, dt)}
         out["batched_vs_cpu"] = value / rate

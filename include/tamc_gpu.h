@@ -236,6 +236,9 @@ typedef struct {
   double prep_ms;            /* host time from the call to the first sweep launch (planning,
                                 allocation, uploads, state initialisation) */
   double post_ms;            /* host time after the last sweep (downloads, result assembly) */
+  double groups_busy_ms;     /* sum over work groups of their own phase time: divided by
+                                device_ms times the compute units it gives how busy the mean
+                                compute unit was (one group per compute unit) */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

@@ -77,7 +77,7 @@ class BatchStats(C.Structure):
     _fields_ = [("device_ms", C.c_double), ("wall_ms", C.c_double), ("attempts", C.c_uint64),
                 ("waves_per_group", C.c_uint32), ("sweeps_per_launch", C.c_uint32), ("launches", C.c_uint32),
                 ("groups", C.c_uint32), ("phase_ms", C.c_double * 7), ("icm_passes", C.c_double),
-                ("prep_ms", C.c_double), ("post_ms", C.c_double)]
+                ("prep_ms", C.c_double), ("post_ms", C.c_double), ("groups_busy_ms", C.c_double)]
 
 
 class SaParams(C.Structure):

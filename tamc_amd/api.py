@@ -363,7 +363,8 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
     st = dict(device_ms=stats.device_ms, wall_ms=stats.wall_ms, attempts=stats.attempts,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
-              icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms)
+              icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms,
+              groups_busy_ms=stats.groups_busy_ms)
     return res, st
 
 

@@ -553,7 +553,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   }
   if (stats) {
     // phase split of the group that took longest
-    unsigned long long best[kPhaseSlots] = {0, 0, 0, 0, 0, 0, 0, 0}, best_tot = 0;
+    unsigned long long best[kPhaseSlots] = {0, 0, 0, 0, 0, 0, 0, 0}, best_tot = 0, all_tot = 0;
     for (auto &S : solves)
       for (auto &g : S.groups) {
         unsigned long long tk[kPhaseSlots];
@@ -562,6 +562,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
         memcpy(tk, mirror.host + (src - mirror.dev), sizeof tk);
         unsigned long long tot = 0;
         for (uint32_t k = 0; k < kPhIcmPasses; ++k) tot += tk[k];
+        all_tot += tot;
         if (tot > best_tot) {
           best_tot = tot;
           memcpy(best, tk, sizeof tk);
@@ -569,6 +570,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
       }
     for (uint32_t k = 0; k < kPhIcmPasses; ++k) stats->phase_ms[k] = (double)best[k] * 1e-5;  // 100 MHz ticks
     stats->icm_passes = (double)best[kPhIcmPasses];
+    stats->groups_busy_ms = (double)all_tot * 1e-5;
     stats->device_ms = ms;
     stats->wall_ms = wall_us / 1000.0;
     stats->attempts = attempts;
@@ -790,6 +792,7 @@ tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts,
     acc.attempts += one.attempts;
     acc.launches += one.launches;
     acc.groups += one.groups;
+    acc.groups_busy_ms += one.groups_busy_ms;
   }
   if (stats) *stats = acc;
   return TAMC_OK;

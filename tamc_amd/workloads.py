@@ -11,16 +11,9 @@ import numpy as np
 def lattice_2d(L, rng=None, pm_j=True, ferro=False):
     """Periodic L x L square lattice; bonds (i -> right) and (i -> down) (row-major sites)."""
     n = L * L
-    i0, j0 = [], []
-    for y in range(L):
-        for x in range(L):
-            s = y * L + x
-            i0.append(s)
-            j0.append(y * L + (x + 1) % L)
-            i0.append(s)
-            j0.append(((y + 1) % L) * L + x)
-    i0 = np.array(i0, np.uint32)
-    j0 = np.array(j0, np.uint32)
+    y, x = np.divmod(np.arange(n), L)
+    i0 = np.repeat(np.arange(n), 2).astype(np.uint32)  # site s: (s, right of s), then (s, below s)
+    j0 = np.stack([y * L + (x + 1) % L, ((y + 1) % L) * L + x], axis=1).ravel().astype(np.uint32)
     if ferro:
         K = -np.ones(len(i0))
     elif pm_j:
