@@ -335,6 +335,16 @@ tamc_status tamc_sa_run_batch(uint32_t count, const tamc_instance *const *insts,
                               tamc_sa_result **results, tamc_batch_stats *stats);
 void tamc_sa_result_free(tamc_sa_result *r);
 
+/* SA requests to a tamc_server (SaRunner::run / run_parallel, src/sa.rs:110-165), served beside its
+ * PT-ICM requests: the restart waves of every resident SA solve advance by whole sweeps of their
+ * schedules in each slice, and their spins stay on the device in between.  Each result is identical to tamc_sa_run on the
+ * same (instance, params, seed).  Poll an SA ticket with tamc_server_poll_sa (tamc_server_poll
+ * refuses it, and the other way round); free its result with tamc_sa_result_free. */
+tamc_status tamc_server_submit_sa(tamc_server *srv, const tamc_instance *inst, const tamc_sa_params *params,
+                                  uint64_t *ticket);
+tamc_status tamc_server_poll_sa(tamc_server *srv, uint64_t ticket, int32_t wait_ms, tamc_sa_result **out,
+                                tamc_request_times *times);
+
 /* No counterpart in the reference: tamc_sa_run for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
  * the same kernel and gives the same results as tamc_sa_run, and calls no hipMalloc, hipFree or

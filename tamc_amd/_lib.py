@@ -161,6 +161,9 @@ SIGNATURES = {
     "tamc_server_submit": (C.c_int, [_vp, _vp, C.POINTER(PtParams), _i8p, _u64p]),
     "tamc_server_poll": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.POINTER(C.POINTER(PtResult)),
                                    C.POINTER(RequestTimes)]),
+    "tamc_server_submit_sa": (C.c_int, [_vp, _vp, C.POINTER(SaParams), _u64p]),
+    "tamc_server_poll_sa": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.POINTER(C.POINTER(SaResult)),
+                                      C.POINTER(RequestTimes)]),
     "tamc_server_cancel": (C.c_int, [_vp, C.c_uint64]),
     "tamc_server_get_stats": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "tamc_server_destroy": (None, [_vp]),

@@ -497,7 +497,7 @@ def run_gla(instance, pt_params, partitions=None, partition_file=None, output_fi
 
 
 class Server:
-    """Continuous-batching PT-ICM request loop (tamc_server_* in the C header).
+    """Continuous-batching PT-ICM and SA request loop (tamc_server_* in the C header).
 
     submit() queues a solve and returns a ticket; poll() returns (result, times) once it has
     finished, or None while it runs.  Instances must outlive their requests."""
@@ -534,6 +534,32 @@ class Server:
 
     def wait(self, ticket):
         return self.poll(ticket, wait_ms=-1)
+
+    def submit_sa(self, instance, params):
+        """Queue an SA solve (SaRunner::run, src/sa.rs:110-165); returns its ticket."""
+        betas = params.betas()
+        P = _CSa(len(betas), ptr(betas, _f32p), params.num_replicas, params.threads, params.seed)
+        t = C.c_uint64()
+        check(lib().tamc_server_submit_sa(self._h, instance._h, C.byref(P), C.byref(t)))
+        self._keep[t.value] = instance
+        return t.value
+
+    def poll_sa(self, ticket, wait_ms=0):
+        out = C.POINTER(SaResult)()
+        tm = RequestTimes()
+        st = lib().tamc_server_poll_sa(self._h, ticket, int(wait_ms), C.byref(out), C.byref(tm))
+        if st == TAMC_E_PENDING:
+            return None
+        self._keep.pop(ticket, None)
+        check(st)
+        try:
+            res = _sa_dict(out.contents)
+        finally:
+            lib().tamc_sa_result_free(out)
+        return res, dict(queued_ms=tm.queued_ms, total_ms=tm.total_ms, slices=tm.slices)
+
+    def wait_sa(self, ticket):
+        return self.poll_sa(ticket, wait_ms=-1)
 
     def cancel(self, ticket):
         check(lib().tamc_server_cancel(self._h, ticket))

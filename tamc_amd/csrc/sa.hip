@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "runtime_common.hpp"
+#include "sa_host.hpp"
 #include "tamc_math.hpp"
 
 #define TAMC_EXC_DECL static __device__ const
@@ -33,18 +34,6 @@
 #define TAMC_AS4 __attribute__((address_space(4)))
 
 namespace tamc {
-
-struct SaTask {
-  const uint32_t *info;     // [n][4] per site: first block, blocks, bias bits, row length
-  const uint32_t *blocks;   // [blocks][16]: eight neighbour byte offsets (8 * col), eight coupling bits
-  const float *betas;
-  float offset;
-  uint32_t n, K, R, rbase;  // rbase: first restart of this wave
-  uint64_t seed;
-  uint32_t *spins_out;      // [n][2] bit-plane halves of this wave's restarts (null: states_out)
-  float *energy_out;        // [64]; with states_out: this wave's entries of a [R] array
-  int8_t *states_out;       // null, or this wave's rows of a [R][n] int8 array (+1/-1)
-};
 
 template <class T>
 __device__ __forceinline__ const TAMC_AS4 T *sa_c4(const T *p) {
@@ -97,6 +86,11 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
   auto put = [&](uint32_t i, uint64_t m) {
     if (sh == 0u) sp[i * 2u + half] = half ? (uint32_t)(m >> 32) : (uint32_t)m;
   };
+  if (X.k0 > 0u) {
+    // continue from the plane words the previous launch left (a wave's LDS writes and reads stay
+    // in order, so no barrier is needed)
+    for (uint32_t w = lane; w < n * 2u; w += 64u) sp[w] = X.spins_out[w];
+  } else {
   // initial states: rand_ising_state with the production init stream (one Philox block per quad)
   for (uint32_t q = 0; q < nq; ++q) {
     const U4 o = philox4x32_10(0u, q, r, kTagInit, key0, key1);
@@ -109,11 +103,12 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
       put(i, __builtin_amdgcn_ballot_w64(bit != 0u));
     }
   }
+  }
   // pipeline: site i is processed from rb while the next site's first chunk and the info of the
   // site after it are in flight
   W4 inf_c = ginfo[0], inf_n = ginfo[n > 1u ? 1u : 0u];
   rb[lane] = gblk[(size_t)inf_c.x * 4u + lane];
-  for (uint32_t k = 0; k < X.K; ++k) {
+  for (uint32_t k = X.k0; k < X.k1; ++k) {
     const float beta = sa_c4(X.betas)[k];
     for (uint32_t q = 0; q < nq; ++q) {
       const U4 u = philox4x32_10(k, q, r, kTagSweep, key0, key1);
@@ -179,6 +174,10 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
         inf_n = inf_nn;
       }
     }
+  }
+  if (X.k1 < X.K) {  // more sweeps follow in a later launch: keep the planes
+    for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
+    return;
   }
   // AnnealMinResults::apply_measurements: energy_ref of every final state (src/sa.rs:55-64), the
   // real row entries in the reference's order
@@ -255,6 +254,61 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 // grid of the launch: 8 * ceil(groups / 8) blocks, so every logical group has a block
 inline uint32_t sa_grid(uint32_t ngroups) { return 8u * ((ngroups + 7u) / 8u); }
 
+tamc_status sa_validate(const tamc_instance *inst, const tamc_sa_params &P) {
+  if (!inst || P.num_betas < 1 || !P.betas) return fail(TAMC_E_INVALID_ARG, "invalid SA parameters");
+  if (P.num_replicas > 1 && P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
+  for (uint32_t k = 1; k < P.num_betas; ++k)
+    if (!(P.betas[k] - P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+  if (!sa_wave_lds_words(inst->n)) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
+  return TAMC_OK;
+}
+
+uint32_t sa_wave_lds_words(uint32_t n) {
+  const uint32_t w = sa_wave_words(n);
+  return w * 4u <= 160u * 1024u ? w : 0u;
+}
+
+// waves per work group for instances of up to maxn sites: four when their LDS fits
+static uint32_t sa_waves_per_group(uint32_t wpw) {
+  uint32_t wpg = 4;
+  while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
+  return wpg;
+}
+
+template <class K>
+static tamc_status sa_lds_attribute(K kernel, uint32_t lds) {
+  if (lds > 65536u) TAMC_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return TAMC_OK;
+}
+
+tamc_status sa_launch(const SaTask *d_tasks, uint32_t ntasks, uint32_t maxn, hipStream_t s) {
+  if (!ntasks) return TAMC_OK;
+  const uint32_t wpw = sa_wave_words(maxn), wpg = sa_waves_per_group(wpw), lds = wpg * wpw * 4u;
+  if (tamc_status st = sa_lds_attribute(sa_kernel, lds)) return st;
+  const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
+  hipLaunchKernelGGL(sa_kernel, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, d_tasks, ntasks, wpw, ngroups);
+  TAMC_HIP(hipGetLastError());
+  return TAMC_OK;
+}
+
+tamc_sa_result *sa_assemble(uint32_t n, uint32_t R, const uint32_t *sp, const float *en) {
+  tamc_sa_result *res = (tamc_sa_result *)calloc(1, sizeof(tamc_sa_result));
+  res->n = n;
+  res->R = R;
+  res->energies = (float *)malloc(std::max<size_t>(1, (size_t)R * 4));
+  res->final_states = (int8_t *)malloc(std::max<size_t>(1, (size_t)R * n));
+  for (uint32_t r = 0; r < R; ++r) {
+    const uint32_t w = r / 64, l = r % 64;
+    res->energies[r] = en[(size_t)w * 64 + l];
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t bit = (sp[((size_t)w * n + i) * 2 + (l >> 5)] >> (l & 31)) & 1u;
+      res->final_states[(size_t)r * n + i] = bit ? -1 : 1;
+    }
+    if (r == 0 || res->energies[r] < res->min_energy) res->min_energy = res->energies[r];
+  }
+  return res;
+}
+
 }  // namespace tamc
 
 using namespace tamc;
@@ -274,12 +328,8 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
   Arena A;
   uint32_t maxn = 0, ntasks = 0;
   for (auto &S : solves) {
-    if (!S.inst || S.P.num_betas < 1 || !S.P.betas) return fail(TAMC_E_INVALID_ARG, "invalid SA parameters");
-    if (S.P.num_replicas > 1 && S.P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
-    for (uint32_t k = 1; k < S.P.num_betas; ++k)
-      if (!(S.P.betas[k] - S.P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
-    // run_parallel keeps threads * (num_replicas / threads) restarts (src/sa.rs:147,161)
-    S.R = S.P.num_replicas > 1 ? S.P.threads * (S.P.num_replicas / S.P.threads) : S.P.num_replicas;
+    if (tamc_status st = sa_validate(S.inst, S.P)) return st;
+    S.R = sa_kept_replicas(S.P);
     S.waves = (S.R + 63) / 64;
     maxn = std::max(maxn, S.inst->n);
     ntasks += S.waves;
@@ -287,10 +337,6 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     A.want(&S.d_spins, (size_t)S.waves * S.inst->n * 2);
     A.want(&S.d_energy, (size_t)S.waves * 64);
   }
-  const uint32_t wpw = sa_wave_words(maxn);  // plane words and the row buffer of one wave
-  uint32_t wpg = 4;  // waves per work group
-  while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
-  if (wpw * 4u > 160u * 1024u) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
   SaTask *d_tasks;
   A.want(&d_tasks, ntasks);
   TAMC_HIP(A.commit());
@@ -310,28 +356,21 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
       t.K = S.P.num_betas;
       t.R = S.R;
       t.rbase = w * 64;
+      t.k0 = 0;
+      t.k1 = S.P.num_betas;
       t.seed = S.P.seed;
       t.spins_out = S.d_spins + (size_t)w * S.inst->n * 2;
       t.energy_out = S.d_energy + (size_t)w * 64;
       tasks.push_back(t);
     }
   }
-  TAMC_HIP(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SaTask), hipMemcpyHostToDevice));
-  static uint32_t attr_bytes = 65536u;
-  const uint32_t lds = wpg * wpw * 4u;
-  if (lds > attr_bytes) {
-    TAMC_HIP(hipFuncSetAttribute((const void *)sa_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_bytes = lds;
-  }
+  if (!tasks.empty()) TAMC_HIP(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SaTask), hipMemcpyHostToDevice));
   hipEvent_t e0, e1;
   TAMC_HIP(hipEventCreate(&e0));
   TAMC_HIP(hipEventCreate(&e1));
   const auto w0 = std::chrono::steady_clock::now();
   TAMC_HIP(hipEventRecord(e0, 0));
-  const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
-  hipLaunchKernelGGL(sa_kernel, dim3(sa_grid(ngroups)), dim3(64 * wpg), wpg * wpw * 4u, 0, d_tasks, ntasks, wpw,
-                     ngroups);
-  TAMC_HIP(hipGetLastError());
+  if (tamc_status st = sa_launch(d_tasks, ntasks, maxn, 0)) return st;
   TAMC_HIP(hipEventRecord(e1, 0));
   TAMC_HIP(hipEventSynchronize(e1));
   const auto w1 = std::chrono::steady_clock::now();
@@ -347,20 +386,7 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     std::vector<float> en((size_t)S.waves * 64);
     TAMC_HIP(hipMemcpy(sp.data(), S.d_spins, sp.size() * 4, hipMemcpyDeviceToHost));
     TAMC_HIP(hipMemcpy(en.data(), S.d_energy, en.size() * 4, hipMemcpyDeviceToHost));
-    tamc_sa_result *R = (tamc_sa_result *)calloc(1, sizeof(tamc_sa_result));
-    R->n = n;
-    R->R = S.R;
-    R->energies = (float *)malloc(std::max<size_t>(1, S.R * 4));
-    R->final_states = (int8_t *)malloc(std::max<size_t>(1, (size_t)S.R * n));
-    for (uint32_t r = 0; r < S.R; ++r) {
-      const uint32_t w = r / 64, l = r % 64;
-      R->energies[r] = en[(size_t)w * 64 + l];
-      for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t bit = (sp[((size_t)w * n + i) * 2 + (l >> 5)] >> (l & 31)) & 1u;
-        R->final_states[(size_t)r * n + i] = bit ? -1 : 1;
-      }
-      if (r == 0 || R->energies[r] < R->min_energy) R->min_energy = R->energies[r];
-    }
+    tamc_sa_result *R = sa_assemble(n, S.R, sp.data(), en.data());
     R->attempts = (uint64_t)S.P.num_betas * n * S.R;
     R->device_ms = ms;
     R->timing_us = std::chrono::duration<double, std::micro>(w1 - w0).count();
@@ -371,7 +397,7 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     stats->device_ms = ms;
     stats->wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
     stats->attempts = attempts;
-    stats->waves_per_group = wpg;
+    stats->waves_per_group = sa_waves_per_group(sa_wave_words(maxn));
     stats->launches = 1;
     stats->groups = ntasks;
   }
@@ -387,16 +413,11 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   clear_error();
   if (!inst || !params || !params->betas || !energies || !states) return fail(TAMC_E_INVALID_ARG, "null argument");
   const tamc_sa_params &P = *params;
-  if (P.num_betas < 1) return fail(TAMC_E_INVALID_ARG, "invalid SA parameters");
-  if (P.num_replicas > 1 && P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
-  for (uint32_t k = 1; k < P.num_betas; ++k)
-    if (!(P.betas[k] - P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
+  if (tamc_status st = sa_validate(inst, P)) return st;
   if ((size_t)P.num_betas * 4u > kNoAllocScratchBytes)
     return fail(TAMC_E_INVALID_ARG, "schedule longer than the allocation-free scratch holds (131072 betas)");
   const uint32_t n = inst->n;
-  if (sa_wave_words(n) * 4u > 160u * 1024u)
-    return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
-  const uint32_t R = P.num_replicas > 1 ? P.threads * (P.num_replicas / P.threads) : P.num_replicas;
+  const uint32_t R = sa_kept_replicas(P);
   tamc_status st = device_init();
   if (st) return st;
   const DeviceInstance *D = resident_instance(inst);
@@ -433,12 +454,11 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   t.seed = P.seed;
   t.energy_out = energies;
   t.states_out = states;
-  const uint32_t ntasks = (R + 63) / 64, wpw = sa_wave_words(n);
-  uint32_t wpg = 4;
-  while (wpg > 1 && wpg * wpw * 4u > 160u * 1024u) wpg >>= 1;
+  t.k0 = 0;
+  t.k1 = P.num_betas;
+  const uint32_t ntasks = (R + 63) / 64, wpw = sa_wave_words(n), wpg = sa_waves_per_group(wpw);
   const uint32_t lds = wpg * wpw * 4u;
-  if (lds > 65536u)
-    TAMC_HIP(hipFuncSetAttribute((const void *)sa_kernel_direct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (tamc_status e = sa_lds_attribute(sa_kernel_direct, lds)) return e;
   const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
   hipLaunchKernelGGL(sa_kernel_direct, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, t, ntasks, wpw, ngroups);
   TAMC_HIP(hipGetLastError());

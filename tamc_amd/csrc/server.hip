@@ -1,4 +1,5 @@
-// server.hip -- continuous-batching request loop for PT-ICM solves (tamc_server_* in tamc_gpu.h).
+// server.hip -- continuous-batching request loop for PT-ICM and SA solves (tamc_server_* in
+// tamc_gpu.h).
 //
 // The reference's callers run many PtIcmRunner solves at once from rayon (src/gla.rs:152-162,
 // src/bin/tamc-pt-opt.rs:68-71); here one loop thread owns the GPU and serves them all:
@@ -14,6 +15,10 @@
 //    Group g runs sweeps [t_done, t_done + S_g) with S_g sized from its measured per-sweep time
 //    so that every stream takes about `slice_ms` (stretched to a whole number of sweeps of the
 //    slowest group); requests that arrive meanwhile join at the next slice boundary;
+//  * SA requests: the waves of every resident SA solve run sweeps [done, done + S) of their
+//    schedules in one sa_kernel launch per slice (on stream 1, after that stream's PT buckets);
+//    between slices their bit planes stay in the pool block, and S is sized from a per-entry
+//    cost calibrated on the previous launches;
 //  * completion: results of requests that finished in slice k are assembled while slice k+1 runs.
 //
 // Results depend only on (instance, params, seed): the Philox streams are keyed by sweep, site
@@ -32,6 +37,7 @@
 #include <unordered_map>
 
 #include "pt_host.hpp"
+#include "sa_host.hpp"
 
 namespace tamc {
 namespace {
@@ -109,11 +115,36 @@ struct Request {
   Clock::time_point t_submit, t_admit, t_ready;
   uint32_t slices = 0;
   const tamc_instance *inst = nullptr;  // holds one of inst->server_uses until the request ends
+  // SA requests (tamc_server_submit_sa); their PT plan S stays empty
+  bool sa = false;
+  tamc_sa_params sa_P{};
+  std::vector<float> sa_betas;
+  uint32_t sa_n = 0, sa_R = 0, sa_waves = 0, sa_done = 0, sa_slice = 0;
+  double sa_cost = 0.0;  // row entries a wave walks per sweep
+  const DeviceInstance *sa_dev = nullptr;
+  float *d_sa_betas = nullptr, *d_sa_energy = nullptr;
+  uint32_t *d_sa_spins = nullptr;
+  tamc_sa_result *sa_result = nullptr;
   void drop_instance() {
     if (inst) inst->server_uses.fetch_sub(1);
     inst = nullptr;
   }
-  ~Request() { drop_instance(); }
+  ~Request() {
+    drop_instance();
+    if (sa_result) tamc_sa_result_free(sa_result);
+  }
+  // waves, LDS bytes and work groups the request holds while resident
+  uint32_t hold_waves() const { return sa ? sa_waves : waves * (uint32_t)S.groups.size(); }
+  uint64_t hold_lds() const {
+    return sa ? (uint64_t)sa_waves * sa_wave_lds_words(sa_n) * 4u : (uint64_t)lds_class * S.groups.size();
+  }
+  size_t hold_groups() const { return sa ? (sa_waves + 3u) / 4u : S.groups.size(); }
+  bool finished_all() const {
+    if (sa) return sa_done >= sa_P.num_betas;
+    for (uint32_t gi = 0; gi < S.groups.size(); ++gi)
+      if (t_done[gi] < S.T) return false;
+    return true;
+  }
 };
 
 // Pinned host + device staging for one launch bucket.
@@ -139,6 +170,11 @@ struct tamc_server {
   uint32_t *h_ranges[kStreams] = {}, *d_ranges[kStreams] = {};
   unsigned long long *h_ticks[kStreams] = {}, *d_ticks[kStreams] = {};
   PtGroup *h_init_groups = nullptr, *d_init_groups = nullptr;
+  // SA tasks of one slice (stream 1), and the events around their launch
+  SaTask *h_sa_tasks = nullptr, *d_sa_tasks = nullptr;
+  hipEvent_t ev_sa0 = nullptr, ev_sa1 = nullptr;
+  std::vector<std::pair<Request *, uint32_t>> sa_batch;  // (request, sweeps) of the running slice
+  double sa_us_per_entry = 0.02;  // calibrated launch time per row entry walked by one wave
   const uint32_t **h_init_ptrs = nullptr;
   const uint32_t **d_init_ptrs = nullptr;
   hipStream_t streams[kStreams] = {};
@@ -174,6 +210,10 @@ struct tamc_server {
       if (ev_end[k]) (void)hipEventDestroy(ev_end[k]);
     }
     if (h_init_groups) (void)hipHostFree(h_init_groups);
+    if (h_sa_tasks) (void)hipHostFree(h_sa_tasks);
+    if (d_sa_tasks) (void)hipFree(d_sa_tasks);
+    if (ev_sa0) (void)hipEventDestroy(ev_sa0);
+    if (ev_sa1) (void)hipEventDestroy(ev_sa1);
     if (h_init_ptrs) (void)hipHostFree((void *)h_init_ptrs);
     if (d_init_groups) (void)hipFree(d_init_groups);
     if (d_init_ptrs) (void)hipFree((void *)d_init_ptrs);
@@ -218,6 +258,10 @@ struct tamc_server {
     TAMC_HIP(hipHostMalloc((void **)&h_init_ptrs, max_groups * sizeof(void *), hipHostMallocDefault));
     TAMC_HIP(hipMalloc((void **)&d_init_groups, max_groups * sizeof(PtGroup)));
     TAMC_HIP(hipMalloc((void **)&d_init_ptrs, max_groups * sizeof(void *)));
+    TAMC_HIP(hipHostMalloc((void **)&h_sa_tasks, opt.max_wave_slots * sizeof(SaTask), hipHostMallocDefault));
+    TAMC_HIP(hipMalloc((void **)&d_sa_tasks, opt.max_wave_slots * sizeof(SaTask)));
+    TAMC_HIP(hipEventCreate(&ev_sa0));
+    TAMC_HIP(hipEventCreate(&ev_sa1));
     return TAMC_OK;
   }
 
@@ -245,14 +289,18 @@ struct tamc_server {
         finish_cancel(r);
         continue;
       }
-      const uint32_t need_waves = r->waves * (uint32_t)r->S.groups.size();
-      const uint64_t need_lds = (uint64_t)r->lds_class * r->S.groups.size();
+      const uint32_t need_waves = r->hold_waves();
+      const uint64_t need_lds = r->hold_lds();
       const bool room = resident.empty() || (resident_waves + need_waves <= opt.max_wave_slots &&
                                              lds_in_use + need_lds <= (uint64_t)cus * kLdsClasses[2] &&
-                                             resident_group_count() + r->S.groups.size() <= max_groups);
+                                             resident_group_count() + r->hold_groups() <= max_groups);
       if (!room) break;
       if (!r->in_pool) {
-        if (r->A.reqs.empty()) {
+        if (r->A.reqs.empty() && r->sa) {
+          r->A.want(&r->d_sa_betas, r->sa_P.num_betas);
+          r->A.want(&r->d_sa_spins, (size_t)r->sa_waves * r->sa_n * 2);
+          r->A.want(&r->d_sa_energy, (size_t)r->sa_waves * 64);
+        } else if (r->A.reqs.empty()) {
           want_buffers(r->S, r->A);
           r->init_dev.assign(r->S.groups.size(), nullptr);
           for (size_t gi = 0; gi < r->S.groups.size(); ++gi)
@@ -286,7 +334,7 @@ struct tamc_server {
 
   size_t resident_group_count() const {
     size_t k = 0;
-    for (auto *r : resident) k += r->S.groups.size();
+    for (auto *r : resident) k += r->hold_groups();
     return k;
   }
 
@@ -300,8 +348,8 @@ struct tamc_server {
 
   void leave_resident(Request *r) {
     resident.erase(std::remove(resident.begin(), resident.end(), r), resident.end());
-    resident_waves -= r->waves * (uint32_t)r->S.groups.size();
-    lds_in_use -= (uint64_t)r->lds_class * r->S.groups.size();
+    resident_waves -= r->hold_waves();
+    lds_in_use -= r->hold_lds();
   }
 
   void fail_request(Request *r, tamc_status e, const std::string &m) {
@@ -340,8 +388,17 @@ struct tamc_server {
   tamc_status init_fresh(std::vector<Request *> &fresh) {
     if (tamc_status s = drain_init()) return s;
     uint32_t ng = 0;
-    bool any_init = false;
+    bool any_init = false, any_sa = false;
     for (Request *r : fresh) {
+      if (r->sa) {  // the schedule's betas; the kernel makes the initial states itself
+        TAMC_HIP(hipMemcpyAsync(r->d_sa_betas, r->sa_betas.data(), r->sa_betas.size() * 4, hipMemcpyHostToDevice,
+                                init_stream));
+        r->initialized = true;
+        r->init_pending = true;
+        initializing.push_back(r);
+        any_sa = true;
+        continue;
+      }
       tamc_status s = fill_groups(r->S, init_stream);
       if (s) return s;
       for (size_t gi = 0; gi < r->S.groups.size(); ++gi) {
@@ -359,12 +416,14 @@ struct tamc_server {
       r->init_pending = true;
       initializing.push_back(r);
     }
-    if (!ng) return TAMC_OK;
-    TAMC_HIP(hipMemcpyAsync(d_init_groups, h_init_groups, ng * sizeof(PtGroup), hipMemcpyHostToDevice, init_stream));
-    if (any_init)
-      TAMC_HIP(hipMemcpyAsync((void *)d_init_ptrs, (const void *)h_init_ptrs, ng * sizeof(void *),
-                              hipMemcpyHostToDevice, init_stream));
-    TAMC_HIP(tamc_launch_pt_init(d_init_groups, ng, any_init ? d_init_ptrs : nullptr, init_stream));
+    if (!ng && !any_sa) return TAMC_OK;
+    if (ng) {
+      TAMC_HIP(hipMemcpyAsync(d_init_groups, h_init_groups, ng * sizeof(PtGroup), hipMemcpyHostToDevice, init_stream));
+      if (any_init)
+        TAMC_HIP(hipMemcpyAsync((void *)d_init_ptrs, (const void *)h_init_ptrs, ng * sizeof(void *),
+                                hipMemcpyHostToDevice, init_stream));
+      TAMC_HIP(tamc_launch_pt_init(d_init_groups, ng, any_init ? d_init_ptrs : nullptr, init_stream));
+    }
     TAMC_HIP(hipEventRecord(ev_init, init_stream));
     init_outstanding = true;
     return TAMC_OK;
@@ -408,6 +467,38 @@ struct tamc_server {
         ++k;
       }
     }
+    // SA solves: every resident one runs a whole number of sweeps on stream 1, after that stream's
+    // buckets, within the same share of the slice as each of them
+    sa_batch.clear();
+    uint32_t sa_tasks = 0, sa_maxn = 0;
+    {
+      const double budget_us = target_us / (double)(per_stream[1].size() + 1u);
+      for (Request *r : resident) {
+        if (!r->sa || r->init_pending || r->sa_done >= r->sa_P.num_betas) continue;
+        const double per = std::max(1e-3, sa_us_per_entry * r->sa_cost);
+        const double want = std::floor(budget_us / per + 1e-9);
+        const uint32_t sweeps = (uint32_t)std::min<double>(r->sa_P.num_betas - r->sa_done, std::max(1.0, want));
+        for (uint32_t w = 0; w < r->sa_waves; ++w) {
+          SaTask &t = h_sa_tasks[sa_tasks++];
+          t = SaTask{};
+          t.info = (const uint32_t *)r->sa_dev->sa_info.p;
+          t.blocks = (const uint32_t *)r->sa_dev->sa_blocks.p;
+          t.betas = r->d_sa_betas;
+          t.offset = r->inst->offset;
+          t.n = r->sa_n;
+          t.K = r->sa_P.num_betas;
+          t.R = r->sa_R;
+          t.rbase = w * 64u;
+          t.k0 = r->sa_done;
+          t.k1 = r->sa_done + sweeps;
+          t.seed = r->sa_P.seed;
+          t.spins_out = r->d_sa_spins + (size_t)w * r->sa_n * 2;
+          t.energy_out = r->d_sa_energy + (size_t)w * 64;
+        }
+        sa_maxn = std::max(sa_maxn, r->sa_n);
+        sa_batch.push_back({r, sweeps});
+      }
+    }
     *slice_ms = 0.0;
     bool launched[kStreams] = {false, false, false};
     std::vector<std::vector<std::pair<Bucket *, uint32_t>>> offsets(kStreams);
@@ -428,19 +519,30 @@ struct tamc_server {
         }
         base += (uint32_t)b->members.size();
       }
-      if (!base) continue;
-      TAMC_HIP(hipMemcpyAsync(d_groups[s], h_groups[s], base * sizeof(PtGroup), hipMemcpyHostToDevice, streams[s]));
-      TAMC_HIP(hipMemcpyAsync(d_ranges[s], h_ranges[s], base * 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
-                              streams[s]));
+      const bool sa_here = s == 1u && sa_tasks > 0u;
+      if (!base && !sa_here) continue;
+      if (base) {
+        TAMC_HIP(hipMemcpyAsync(d_groups[s], h_groups[s], base * sizeof(PtGroup), hipMemcpyHostToDevice, streams[s]));
+        TAMC_HIP(hipMemcpyAsync(d_ranges[s], h_ranges[s], base * 2 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                streams[s]));
+      }
+      if (sa_here)
+        TAMC_HIP(hipMemcpyAsync(d_sa_tasks, h_sa_tasks, sa_tasks * sizeof(SaTask), hipMemcpyHostToDevice, streams[s]));
       TAMC_HIP(hipEventRecord(ev_start[s], streams[s]));
       for (auto &bo : offsets[s]) {
         Bucket *b = bo.first;
         TAMC_HIP(tamc_launch_pt_sweeps(d_groups[s] + bo.second, (uint32_t)b->members.size(), 0, 0, b->waves, b->lds, 1u,
                                        d_ranges[s] + 2 * bo.second, d_ticks[s] + bo.second, streams[s]));
       }
+      if (sa_here) {
+        TAMC_HIP(hipEventRecord(ev_sa0, streams[s]));
+        if (tamc_status e = sa_launch(d_sa_tasks, sa_tasks, sa_maxn, streams[s])) return e;
+        TAMC_HIP(hipEventRecord(ev_sa1, streams[s]));
+      }
       TAMC_HIP(hipEventRecord(ev_end[s], streams[s]));
-      TAMC_HIP(hipMemcpyAsync(h_ticks[s], d_ticks[s], base * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              streams[s]));
+      if (base)
+        TAMC_HIP(hipMemcpyAsync(h_ticks[s], d_ticks[s], base * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                streams[s]));
       launched[s] = true;
     }
     slice_launched = launched[0] || launched[1] || launched[2];
@@ -485,10 +587,56 @@ struct tamc_server {
         }
       }
     }
+    if (!sa_batch.empty()) {
+      // the launch lasts as long as its longest wave: recalibrate the time per row entry from it
+      float ms = 0.f;
+      TAMC_HIP(hipEventElapsedTime(&ms, ev_sa0, ev_sa1));
+      double longest = 0.0;
+      for (auto &rs : sa_batch) longest = std::max(longest, rs.first->sa_cost * rs.second);
+      if (longest > 0.0 && ms > 0.f) sa_us_per_entry = 0.5 * (sa_us_per_entry + (double)ms * 1000.0 / longest);
+      for (auto &rs : sa_batch) rs.first->sa_done += rs.second;
+    }
     return TAMC_OK;
   }
 
+  void complete_sa(Request *r) {
+    const Clock::time_point t0 = Clock::now();
+    const size_t np = (size_t)r->sa_waves * r->sa_n * 2, ne = (size_t)r->sa_waves * 64;
+    std::vector<uint32_t> planes(np);
+    std::vector<float> en(ne);
+    tamc_sa_result *res = nullptr;
+    if (hipMemcpyAsync(planes.data(), r->d_sa_spins, np * 4, hipMemcpyDeviceToHost, copy_stream) == hipSuccess &&
+        hipMemcpyAsync(en.data(), r->d_sa_energy, ne * 4, hipMemcpyDeviceToHost, copy_stream) == hipSuccess &&
+        hipStreamSynchronize(copy_stream) == hipSuccess)
+      res = sa_assemble(r->sa_n, r->sa_R, planes.data(), en.data());
+    std::lock_guard<std::mutex> lk(mu);
+    st.complete_ms += ms_between(t0, Clock::now());
+    if (!res) {
+      fail_request(r, TAMC_E_DEVICE, "result download failed");
+      return;
+    }
+    release_resources(r);
+    res->attempts = (uint64_t)r->sa_P.num_betas * r->sa_n * r->sa_R;
+    res->timing_us = ms_between(r->t_admit, Clock::now()) * 1000.0;
+    r->sa_result = res;
+    r->state = State::Done;
+    r->t_ready = Clock::now();
+    st.completed++;
+    st.attempts += res->attempts;
+    if (r->cancel) {
+      tamc_sa_result_free(res);
+      r->sa_result = nullptr;
+      finish_cancel(r);
+      return;
+    }
+    cv_done.notify_all();
+  }
+
   void complete(Request *r) {
+    if (r->sa) {
+      complete_sa(r);
+      return;
+    }
     const Clock::time_point t0 = Clock::now();
     tamc_pt_result *res = nullptr;
     // one bulk download of the request's pool block, then assembly from the host copy
@@ -586,9 +734,7 @@ struct tamc_server {
       for (Request *r : std::vector<Request *>(resident)) {
         if (r->init_pending) continue;
         r->slices++;
-        bool all_done = true;
-        for (uint32_t gi = 0; gi < r->S.groups.size(); ++gi) all_done &= r->t_done[gi] >= r->S.T;
-        if (all_done) {
+        if (r->finished_all()) {
           leave_resident(r);
           r->state = State::Finished;
           finished_prev.push_back(r);
@@ -651,15 +797,50 @@ tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, cons
   return TAMC_OK;
 }
 
-tamc_status tamc_server_poll(tamc_server *srv, uint64_t ticket, int32_t wait_ms, tamc_pt_result **out,
-                             tamc_request_times *times) {
+tamc_status tamc_server_submit_sa(tamc_server *srv, const tamc_instance *inst, const tamc_sa_params *params,
+                                  uint64_t *ticket) {
   clear_error();
-  if (!srv || !out) return fail(TAMC_E_INVALID_ARG, "null argument");
-  *out = nullptr;
+  if (!srv || !ticket || !params) return fail(TAMC_E_INVALID_ARG, "null argument");
+  if (tamc_status s = sa_validate(inst, *params)) return s;
+  std::unique_ptr<Request> r(new Request());
+  r->t_submit = Clock::now();
+  inst->server_uses.fetch_add(1);
+  r->inst = inst;
+  r->sa = true;
+  r->sa_P = *params;
+  r->sa_betas.assign(params->betas, params->betas + params->num_betas);
+  r->sa_P.betas = r->sa_betas.data();
+  r->sa_n = inst->n;
+  r->sa_R = sa_kept_replicas(*params);
+  r->sa_waves = (r->sa_R + 63u) / 64u;
+  r->sa_cost = (double)inst->col.size() + inst->n;  // row entries plus per-site work of one sweep
+  if (r->sa_waves > srv->opt.max_wave_slots)
+    return fail(TAMC_E_UNSUPPORTED, "request has more restart waves than the server's wave slots");
+  tamc_status s = TAMC_OK;
+  // the instance and its SA rows are uploaded (once per instance) here, in the caller's thread
+  if (!(r->sa_dev = device_sa_rows(inst, &s))) return s;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (srv->stop) return fail(TAMC_E_INVALID_ARG, "server is shutting down");
+  r->id = srv->next_id++;
+  *ticket = r->id;
+  srv->queue.push_back(r.get());
+  srv->all[r->id] = std::move(r);
+  srv->st.submitted++;
+  srv->cv_work.notify_one();
+  return TAMC_OK;
+}
+
+// the common part of tamc_server_poll and tamc_server_poll_sa: waits for the request (of kind sa)
+// and hands back its record, or returns TAMC_E_PENDING / the request's error
+static tamc_status poll_request(tamc_server *srv, uint64_t ticket, int32_t wait_ms, bool sa,
+                                tamc_request_times *times, std::unique_ptr<Request> *taken) {
   std::unique_lock<std::mutex> lk(srv->mu);
   auto it = srv->all.find(ticket);
   if (it == srv->all.end()) return fail(TAMC_E_INVALID_ARG, "unknown or already collected ticket");
   Request *r = it->second.get();
+  if (r->sa != sa)
+    return fail(TAMC_E_INVALID_ARG, sa ? "ticket is a PT-ICM request: poll it with tamc_server_poll"
+                                       : "ticket is an SA request: poll it with tamc_server_poll_sa");
   auto ready = [&] { return r->state == State::Done || r->state == State::Failed; };
   if (!ready() && wait_ms != 0) {
     if (wait_ms < 0) srv->cv_done.wait(lk, ready);
@@ -671,13 +852,38 @@ tamc_status tamc_server_poll(tamc_server *srv, uint64_t ticket, int32_t wait_ms,
     times->total_ms = ms_between(r->t_submit, r->t_ready);
     times->slices = r->slices;
   }
-  const tamc_status e = r->err;
-  const std::string m = r->msg;
-  *out = r->result;
-  r->result = nullptr;
+  *taken = std::move(it->second);
   srv->all.erase(it);
-  if (e) return fail(e, m);
+  if ((*taken)->err) return fail((*taken)->err, (*taken)->msg);
   return TAMC_OK;
+}
+
+tamc_status tamc_server_poll(tamc_server *srv, uint64_t ticket, int32_t wait_ms, tamc_pt_result **out,
+                             tamc_request_times *times) {
+  clear_error();
+  if (!srv || !out) return fail(TAMC_E_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<Request> r;
+  const tamc_status e = poll_request(srv, ticket, wait_ms, false, times, &r);
+  if (r && !e) {
+    *out = r->result;
+    r->result = nullptr;
+  }
+  return e;
+}
+
+tamc_status tamc_server_poll_sa(tamc_server *srv, uint64_t ticket, int32_t wait_ms, tamc_sa_result **out,
+                                tamc_request_times *times) {
+  clear_error();
+  if (!srv || !out) return fail(TAMC_E_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<Request> r;
+  const tamc_status e = poll_request(srv, ticket, wait_ms, true, times, &r);
+  if (r && !e) {
+    *out = r->sa_result;
+    r->sa_result = nullptr;
+  }
+  return e;
 }
 
 tamc_status tamc_server_cancel(tamc_server *srv, uint64_t ticket) {

@@ -92,3 +92,43 @@ def test_server_cancel_and_errors(gpu):
         st = srv.stats()
         assert st["cancelled"] == 1 and st["completed"] == len(reqs) - 1
         assert st["pool_used"] == 0 and st["queued"] == 0
+
+
+def _sa_requests():
+    out = []
+    for k, (kind, kw, R, threads, K) in enumerate([("qubo", dict(n=200, density=0.1), 300, 3, 60),
+                                                   ("2d_gauss", dict(L=6), 64, 1, 30),
+                                                   ("qubo", dict(n=2048, density=0.05), 1024, 1, 24),
+                                                   ("rr3", dict(n=70), 1, 1, 40)]):
+        _, gi, _ = make_pair(kind, seed=60 + k, **kw)
+        betas = np.linspace(0.1, 4.0, K).astype(np.float32)
+        out.append((gi, api.SaParams(beta=api.BetaOptions.Arr(betas), num_replicas=R, threads=threads, seed=70 + k)))
+    return out
+
+
+def test_server_sa_requests_equal_single_runs(gpu):
+    # SA solves served beside PT-ICM solves: a short slice target splits each SA schedule over
+    # several launches (the restarts' planes stay on the device in between); every result equals
+    # tamc_sa_run on the same inputs, and the PT results stay those of their single runs
+    sa = _sa_requests()
+    pt = mixed_requests()[:4]
+    sa_alone = [api.SaRunner(i, p).run() for i, p in sa]
+    pt_alone = [api.PtIcmRunner(i, p).run() for i, p in pt]
+    with api.Server(slice_ms=0.5) as srv:
+        ts = [srv.submit_sa(i, p) for i, p in sa[:2]]
+        tp = [srv.submit(i, p) for i, p in pt]
+        ts += [srv.submit_sa(i, p) for i, p in sa[2:]]
+        with pytest.raises(TamcError):
+            srv.poll(ts[0])  # an SA ticket is polled with poll_sa
+        sa_served = [srv.wait_sa(t) for t in ts]
+        pt_served = [srv.wait(t)[0] for t in tp]
+        st = srv.stats()
+    for (res, times), ref in zip(sa_served, sa_alone):
+        assert res["R"] == ref["R"]
+        assert np.array_equal(res["final_states"], ref["final_states"])
+        assert np.array_equal(res["energies"], ref["energies"])
+        assert res["min_energy"] == ref["min_energy"] and res["attempts"] == ref["attempts"]
+    assert sa_served[2][1]["slices"] > 1  # the 2048-site QUBO took several launches
+    for res, ref in zip(pt_served, pt_alone):
+        same(res, ref)
+    assert st["completed"] == len(sa) + len(pt) and st["failed"] == 0 and st["pool_used"] == 0
