@@ -824,7 +824,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     if (total) {
       uint32_t k;
       if (X.icm_k) {
-        k = c4(X.icm_k)[p];
+        k = c4(X.icm_k)[c.lo - X.lo + p];
         if (k >= total) k = total - 1u;
       } else {
         const U4 v = philox4x32_10(t, c.lo + p, X.pair_global, kTagIcm, c.key0, c.key1);
@@ -1454,8 +1454,16 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     asm volatile("" : "+s"(gp), "+s"(il), "+s"(lb));
     Ctx cs = make_ctx(gp, il, lb);
     if ((cs.flags & kFlagIcm) && cs.P > 0) {
-      if (cs.icm_lds) icm_phase<true>(cs, t);
-      else icm_phase<false>(cs, t);
+      // pairs in chunks of at most kMaxPairs slots; a pair's moves depend on its own slot only
+      const uint32_t Pt = cs.P, lo0 = cs.lo;
+      for (uint32_t off = 0; off < Pt; off += kMaxPairs) {
+        cs.lo = lo0 + off;
+        cs.P = icm_chunk(Pt - off);
+        if (cs.icm_lds) icm_phase<true>(cs, t);
+        else icm_phase<false>(cs, t);
+      }
+      cs.lo = lo0;
+      cs.P = Pt;
     }
     lap(cs, kPhIcmFlip);
     if (cs.flags & kFlagIcmOnly) break;

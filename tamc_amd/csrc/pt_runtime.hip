@@ -69,7 +69,6 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
       return fail(TAMC_E_UNSUPPORTED, "a chain pair may hold at most 256 replicas (num_betas <= 128)");
     g.G = (g.R + 63) / 64;
     g.P = (P->icm && g.nch == 2 && P->lo_beta_idx < S.B) ? S.B - P->lo_beta_idx : 0;
-    if (g.P > kMaxPairs) return fail(TAMC_E_UNSUPPORTED, "at most 32 beta slots may take part in ICM");
     const LdsLayout L = make_layout(S.n, g.G, g.R, S.B, g.P, waves, true);
     if (L.total > kLdsLimit) {
       const LdsLayout L2 = make_layout(S.n, g.G, g.R, S.B, g.P, waves, false);
@@ -152,7 +151,7 @@ void want_buffers(SolvePlan &S, Arena &A, Arena *Ags, Arena *Ab) {
     if (g.icm_global) {
       A.want(&X.g_qw, S.n);
       A.want(&X.g_cw, S.n);
-      A.want(&X.g_blk, (size_t)nb * g.P);
+      A.want(&X.g_blk, (size_t)nb * icm_chunk(g.P));
     }
   }
 }

@@ -101,6 +101,10 @@ struct LdsLayout {
 
 __host__ __device__ inline uint32_t lds_align(uint32_t x) { return (x + 15u) & ~15u; }
 
+// ICM runs the pairs of a group in chunks of at most kMaxPairs (one bit per pair in the 32-bit
+// overlap and cluster words); the number of pairs one chunk holds
+__host__ __device__ inline uint32_t icm_chunk(uint32_t P) { return P < kMaxPairs ? P : kMaxPairs; }
+
 // icm_lds: ICM arrays (q words, cluster words, per-block counts) are placed in LDS.
 __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_t R, uint32_t B, uint32_t P,
                                                  uint32_t W, bool icm_lds) {
@@ -154,8 +158,8 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
     o += lds_align(n * 4u);
     L.cw = o;
     o += lds_align(n * 4u);
-    L.blk = o;  // per-pair counts of q<0 sites in each 64-site block, [P][nblk]
-    o += lds_align(nblk * P * 4u);
+    L.blk = o;  // per-pair counts of q<0 sites in each 64-site block, [P][nblk] (one chunk of pairs)
+    o += lds_align(nblk * icm_chunk(P) * 4u);
   } else {
     L.qw = L.cw = L.blk = kNone;
   }

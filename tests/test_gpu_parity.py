@@ -216,6 +216,19 @@ def test_pt_ladder_longer_than_a_wave(gpu, threads):
     assert_same(g, o, exact_energy=True)
 
 
+@pytest.mark.parametrize("kind,kw,B,lo,C,threads,T", [
+    ("3d_pm", dict(L=3), 48, 4, 2, 1, 16),        # 44 ICM slots: chunks of 32 and 12
+    ("2d_pm", dict(L=6), 80, 2, 4, 2, 12),        # 78 slots in three chunks, two chain pairs
+    ("rr3", dict(n=200), 64, 0, 2, 1, 8),         # every slot, Gaussian couplings
+    ("2d_pm", dict(L=88), 40, 0, 2, 1, 3)])       # ICM words in global memory
+def test_icm_more_than_32_slots(gpu, kind, kw, B, lo, C, threads, T):
+    # a cluster word holds 32 pairs: the kernel runs the pairs of a chain pair 32 slots at a time
+    g, o = _run_both(kind, kw, B=B, lo=lo, C=C, threads=threads, T=T, sample=2, sstates=3, limiting=2)
+    assert o["error"] == 0
+    assert g["icm_moves"] > 32 * T
+    assert_same(g, o, exact_energy=True)
+
+
 def _streams(count, length, seed):
     x = O.Xoshiro(seed)
     out = []
