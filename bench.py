@@ -306,7 +306,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch without hipGraph capture")
+    ap.add_argument("--tiling", type=int, default=0, help="task tiling (1 dealt in turn, 2 runs; 0 = autotuned)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the other configurations")
@@ -357,7 +358,7 @@ def main():
 
     def step():
         return api.run_pt_batch(insts, params, waves_per_group=args.waves, sweeps_per_launch=args.sweeps_per_launch,
-                                use_graph=args.graph, autotune=True)
+                                use_graph=not args.no_graph, autotune=True, task_tiling=args.tiling)
 
     for _ in range(args.warmup):
         step()
@@ -407,7 +408,8 @@ def main():
                         % (len(insts), sum_n, args.sweeps),
             "instances": len(insts), "sum_n": sum_n, "betas": 32, "chains": 2, "sweeps": args.sweeps,
             "waves_per_group": st["waves_per_group"], "sweeps_per_launch": st["sweeps_per_launch"],
-            "launches": st["launches"],
+            "launches": st["launches"], "task_tiling": {1: "dealt in turn", 2: "contiguous runs"}[st["task_tiling"]],
+            "hipgraph": st["graph"],
         },
         "kernel_attempts_per_s": kernel_rate,
         # where a step's time outside the kernel goes: library preparation (planning, allocation,

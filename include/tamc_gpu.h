@@ -221,8 +221,15 @@ typedef struct {
                                 instances' level widths */
   uint32_t sweeps_per_launch;/* 0 = all sweeps in one launch */
   int32_t use_graph;         /* capture the launch sequence in a hipGraph */
-  int32_t autotune;          /* time 4, 8 and 16 waves per group on a 24-sweep prefix of the same batch
-                                and keep the fastest (cached per batch shape) */
+  int32_t autotune;          /* choose every setting left at 0 by timing the candidates on a 24-sweep
+                                prefix of the same batch (measurements off) and keeping the fastest;
+                                choices are cached per batch shape.  Candidates: 4, 8 and 16 waves per
+                                group x both task tilings, then one launch for all sweeps against
+                                four launches of a quarter each (captured in a hipGraph) */
+  uint32_t task_tiling;      /* how a colour level's tasks (4-site RNG groups x plane words) are laid
+                                over a group's waves: 1 = dealt to the waves in turn, 2 = one
+                                contiguous run per wave; 0 = tuned with `autotune`, else 1.  Results
+                                do not depend on it */
 } tamc_batch_options;
 
 typedef struct {
@@ -239,6 +246,8 @@ typedef struct {
   double groups_busy_ms;     /* sum over work groups of their own phase time: divided by
                                 device_ms times the compute units it gives how busy the mean
                                 compute unit was (one group per compute unit) */
+  uint32_t task_tiling;      /* tiling used (1 dealt in turn, 2 contiguous runs) */
+  int32_t graph;             /* 1 when the launches ran as a captured hipGraph */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

@@ -1095,9 +1095,18 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   const uint32_t nl = X.n_levels;
   const bool self = X.self_J != nullptr;
   uint32_t a = c4(X.lvl_off)[0];
+  const bool blocks = (c.flags & kFlagTileBlocks) != 0u;
   for (uint32_t l = 0; l < nl; ++l) {
     const uint32_t b = c4(X.lvl_off)[l + 1];
-    for (uint32_t k = a + c.wave; k < b; k += c.W) {
+    // tiling of the level's tasks [a, b) over the waves: dealt in turn, or one contiguous run each
+    uint32_t k = a + c.wave, kend = b, kstep = c.W;
+    if (blocks) {
+      const uint32_t per = (b - a + c.W - 1u) / c.W;
+      k = a + c.wave * per;
+      kend = (k + per < b) ? k + per : b;
+      kstep = 1u;
+    }
+    for (; k < kend; k += kstep) {
       const uint32_t grp = (G == 1u) ? k : k / G, g = k - grp * G;
       const uint32_t r = g * 64u + c.lane;
       const bool valid = r < c.R;
@@ -1448,23 +1457,22 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   __syncthreads();
   const uint32_t tend = t1 < X.T ? t1 : X.T;
   if (c.tid == 0) c.ticks[kPhaseSlots] = wall_clock64();
-  for (uint32_t t = t0; t < tend; ++t) {
+  // one iteration per ICM chunk of at most kMaxPairs slots (a pair's moves depend on its own slot
+  // only); the last chunk's iteration goes on with the rest of the sweep
+  for (uint32_t t = t0, ck = 0; t < tend;) {
     const PtGroup *gp = groups;
     uint32_t il = icm_lds, lb = lds_bytes;
     asm volatile("" : "+s"(gp), "+s"(il), "+s"(lb));
     Ctx cs = make_ctx(gp, il, lb);
     if ((cs.flags & kFlagIcm) && cs.P > 0) {
-      // pairs in chunks of at most kMaxPairs slots; a pair's moves depend on its own slot only
-      const uint32_t Pt = cs.P, lo0 = cs.lo;
-      for (uint32_t off = 0; off < Pt; off += kMaxPairs) {
-        cs.lo = lo0 + off;
-        cs.P = icm_chunk(Pt - off);
-        if (cs.icm_lds) icm_phase<true>(cs, t);
-        else icm_phase<false>(cs, t);
-      }
-      cs.lo = lo0;
-      cs.P = Pt;
+      const uint32_t Pt = cs.P;
+      cs.lo += ck * kMaxPairs;
+      cs.P = icm_chunk(Pt - ck * kMaxPairs);
+      if (cs.icm_lds) icm_phase<true>(cs, t);
+      else icm_phase<false>(cs, t);
+      if ((++ck) * kMaxPairs < Pt) continue;
     }
+    ck = 0;
     lap(cs, kPhIcmFlip);
     if (cs.flags & kFlagIcmOnly) break;
     exchange_phase(cs, t);
@@ -1473,6 +1481,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     lap(cs, kPhSweep);
     measure_phase(cs, t);
     lap(cs, kPhMeasure);
+    ++t;
   }
   __syncthreads();
   if (c.tid == 0 && X.phase_ticks)

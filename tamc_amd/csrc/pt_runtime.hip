@@ -242,7 +242,7 @@ struct Launch {
   }
 };
 
-static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, Launch &Lc) {
+static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, uint32_t tiling, Launch &Lc) {
   struct Item {
     SolvePlan *S;
     GroupPlan *g;
@@ -260,6 +260,7 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
   std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
   for (auto &it : items) {
     Lc.order.push_back(it.g->g);
+    if (tiling == 2u) Lc.order.back().flags |= kFlagTileBlocks;
     if (any_init) {
       if (it.g->init_bits.empty()) {
         Lc.init_ptrs.push_back(nullptr);
@@ -460,6 +461,7 @@ tamc_status assemble(const SolvePlan &S, tamc_pt_result **out, const HostMirror 
 struct RunOptions {
   uint32_t waves = 8;
   uint32_t sweeps_per_launch = 0;  // 0 = all
+  uint32_t tiling = 1;             // 1 tasks dealt to the waves in turn, 2 contiguous runs
   bool use_graph = false;
   std::chrono::steady_clock::time_point t_entry = std::chrono::steady_clock::now();  // for prep_ms
 };
@@ -472,7 +474,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (st) return st;
   if (patch) patch(solves);
   Launch Lc;
-  if ((st = prepare_launch(solves, opt.waves, Lc))) return st;
+  if ((st = prepare_launch(solves, opt.waves, opt.tiling, Lc))) return st;
   const uint32_t ng = (uint32_t)Lc.order.size();
   hipStream_t s;
   TAMC_HIP(hipStreamCreate(&s));
@@ -577,6 +579,8 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
     stats->sweeps_per_launch = spl;
     stats->launches = launches;
     stats->groups = ng;
+    stats->task_tiling = opt.tiling;
+    stats->graph = opt.use_graph ? 1 : 0;
     stats->prep_ms = std::chrono::duration<double, std::milli>(w0 - opt.t_entry).count();
     stats->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w1).count();
   }
@@ -669,9 +673,12 @@ tamc_status tamc_pt_run_resume(const tamc_instance *inst, const tamc_pt_params *
   return run_solves(solves, opt, out, nullptr);
 }
 
-// Autotuner for the batched launch: waves per work group (the task tiling of every level) is
-// chosen by timing each candidate on a short prefix of the same batch (measurements off, results
-// discarded).  Choices are cached per batch shape for the life of the process.
+// Autotuner for the batched launch.  Waves per work group and the task tiling of every colour
+// level are chosen together by timing each candidate on a short prefix of the same batch
+// (measurements off, results discarded); then, on the winner, one launch for the whole prefix is
+// timed against four launches of a quarter each captured in a hipGraph, and the faster choice is
+// scaled to the full run (all sweeps in one launch, or a quarter per launch).  Choices are cached
+// per batch shape for the life of the process.
 namespace {
 struct TuneKey {
   uint32_t count, max_n, B, C, order;
@@ -680,12 +687,17 @@ struct TuneKey {
     return std::tie(count, max_n, B, C, order, sum_n) < std::tie(o.count, o.max_n, o.B, o.C, o.order, o.sum_n);
   }
 };
+struct TuneChoice {
+  uint32_t waves = 16, tiling = 1;
+  bool quarters = false;  // a quarter of the sweeps per launch, launches captured in a hipGraph
+};
 std::mutex g_tune_mu;
-std::map<TuneKey, uint32_t> g_tune_cache;
+std::map<TuneKey, TuneChoice> g_tune_cache;
+constexpr uint32_t kTunePrefix = 24u;
 }  // namespace
 
-static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
-                                  uint32_t *waves_out) {
+static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
+                                  uint32_t fixed_waves, uint32_t fixed_tiling, TuneChoice *out) {
   TuneKey key{count, 0, 0, 0, 0, 0};
   for (uint32_t k = 0; k < count; ++k) {
     key.order |= 1u << params[k].sweep_order;
@@ -694,43 +706,79 @@ static tamc_status autotune_waves(uint32_t count, const tamc_instance *const *in
     key.B = std::max(key.B, params[k].num_betas);
     key.C = std::max(key.C, params[k].num_replica_chains);
   }
+  key.order |= (fixed_waves << 8) | (fixed_tiling << 16);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      *waves_out = it->second;
+      *out = it->second;
       return TAMC_OK;
     }
   }
   std::vector<tamc_pt_params> probe(params, params + count);
   for (auto &p : probe) {
-    p.num_sweeps = std::min(p.num_sweeps, 24u);
+    p.num_sweeps = std::min(p.num_sweeps, kTunePrefix);
     p.warmup_fraction = 1.0;
     p.sample_enabled = 0;
     p.sample_states_enabled = 0;
   }
   std::vector<tamc_pt_result *> tmp(count, nullptr);
-  double best_ms = 1e300;
-  uint32_t best = 16;
-  for (uint32_t w : {4u, 8u, 16u}) {
+  auto time_one = [&](const RunOptions &opt, double *ms) -> tamc_status {
     std::vector<SolvePlan> solves(count);
-    tamc_status st = TAMC_OK;
-    for (uint32_t k = 0; k < count && !st; ++k) st = plan_solve(solves[k], insts[k], &probe[k], nullptr, w);
-    if (st) continue;  // this tiling does not fit (LDS): skip it
-    RunOptions opt;
-    opt.waves = w;
-    tamc_batch_stats bs{};
-    if ((st = run_solves(solves, opt, tmp.data(), &bs))) return st;
-    for (auto *r : tmp) tamc_pt_result_free(r);
-    if (bs.device_ms < best_ms) {
-      best_ms = bs.device_ms;
-      best = w;
+    for (uint32_t k = 0; k < count; ++k) {
+      const tamc_status st = plan_solve(solves[k], insts[k], &probe[k], nullptr, opt.waves);
+      if (st) {
+        *ms = -1.0;  // this shape does not fit (LDS): not a candidate
+        return TAMC_OK;
+      }
     }
+    tamc_batch_stats bs{};
+    const tamc_status st = run_solves(solves, opt, tmp.data(), &bs);
+    if (st) return st;
+    for (auto *r : tmp) tamc_pt_result_free(r);
+    *ms = bs.device_ms;
+    return TAMC_OK;
+  };
+  TuneChoice best;
+  double best_ms = 1e300;
+  bool any = false;
+  for (uint32_t w : {4u, 8u, 16u}) {
+    if (fixed_waves && w != fixed_waves) continue;
+    for (uint32_t tl : {1u, 2u}) {
+      if (fixed_tiling && tl != fixed_tiling) continue;
+      RunOptions opt;
+      opt.waves = w;
+      opt.tiling = tl;
+      double ms;
+      const tamc_status st = time_one(opt, &ms);
+      if (st) return st;
+      if (ms >= 0.0 && ms < best_ms) {
+        best_ms = ms;
+        best.waves = w;
+        best.tiling = tl;
+        any = true;
+      }
+    }
+  }
+  if (!any && fixed_waves) {  // a caller's wave count that the tuner's candidates do not hold
+    best.waves = fixed_waves;
+    best.tiling = fixed_tiling ? fixed_tiling : 1u;
+  }
+  if (any) {
+    RunOptions opt;
+    opt.waves = best.waves;
+    opt.tiling = best.tiling;
+    opt.sweeps_per_launch = (kTunePrefix + 3u) / 4u;
+    opt.use_graph = true;
+    double ms;
+    const tamc_status st = time_one(opt, &ms);
+    if (st) return st;
+    best.quarters = ms >= 0.0 && ms < best_ms;
   }
   clear_error();
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = best;
-  *waves_out = best;
+  *out = best;
   return TAMC_OK;
 }
 
@@ -752,15 +800,25 @@ tamc_status tamc_pt_run_batch_init(uint32_t count, const tamc_instance *const *i
   RunOptions opt;
   uint32_t w = 4;
   for (uint32_t k = 0; k < count; ++k) w = std::max(w, default_waves(insts[k]));
-  if (opts && opts->waves_per_group) {
-    w = opts->waves_per_group;
-  } else if (opts && opts->autotune && count) {
-    if ((st = autotune_waves(count, insts, params, &w))) return st;
-  }
+  if (opts && opts->waves_per_group) w = opts->waves_per_group;
+  if (w < 1 || w > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
+  if (opts && opts->task_tiling > 2u) return fail(TAMC_E_INVALID_ARG, "task_tiling must be 0, 1 or 2");
   opt.waves = w;
-  if (opt.waves < 1 || opt.waves > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
+  opt.tiling = (opts && opts->task_tiling) ? opts->task_tiling : 1u;
   opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
   opt.use_graph = opts && opts->use_graph;
+  if (opts && opts->autotune && count && (!opts->waves_per_group || !opts->task_tiling || !opts->sweeps_per_launch)) {
+    TuneChoice tc;
+    if ((st = autotune_batch(count, insts, params, opts->waves_per_group, opts->task_tiling, &tc))) return st;
+    opt.waves = tc.waves;
+    opt.tiling = tc.tiling;
+    if (!opts->sweeps_per_launch && tc.quarters) {
+      uint32_t maxT = 0;
+      for (uint32_t k = 0; k < count; ++k) maxT = std::max(maxT, params[k].num_sweeps);
+      opt.sweeps_per_launch = std::max(1u, (maxT + 3u) / 4u);
+      opt.use_graph = true;
+    }
+  }
   opt.t_entry = t_entry;  // the autotuner's probe launches count as preparation
   std::vector<SolvePlan> solves(count);
   for (uint32_t k = 0; k < count; ++k)

@@ -29,6 +29,8 @@ enum : uint32_t {
   kFlagSampleStates = 8u,
   kFlagIcmGlobal = 16u,      // ICM scratch lives in global memory (group too large for LDS)
   kFlagIcmOnly = 32u,        // run only the ICM step with seeds given in icm_k (tamc_houdayer)
+  kFlagTileBlocks = 64u,     // sweep tiling: each wave takes one contiguous run of a level's tasks
+                             // (else the tasks are dealt to the waves in turn)
 };
 
 struct PtGroup {

@@ -285,10 +285,14 @@ def test_batch_equals_single_and_plain(gpu):
     plain, _ = api.run_pt_batch(insts, params, plain=True)
     chunked, _ = api.run_pt_batch(insts, params, sweeps_per_launch=7, use_graph=True)
     tuned, tst = api.run_pt_batch(insts, params, autotune=True)
-    assert tst["waves_per_group"] in (4, 8, 16)
+    assert tst["waves_per_group"] in (4, 8, 16) and tst["task_tiling"] in (1, 2)
+    # contiguous task runs per wave, with more waves than some levels have tasks
+    runs, rst = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=2, sweeps_per_launch=10,
+                                 use_graph=True)
+    assert rst["task_tiling"] == 2 and rst["launches"] == 4 and rst["graph"]
     for k in range(len(insts)):
         single = api.PtIcmRunner(insts[k], params[k]).run()
-        for other in (plain[k], chunked[k], tuned[k], single):
+        for other in (plain[k], chunked[k], tuned[k], runs[k], single):
             for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
                 assert np.array_equal(batch[k][key], other[key]), key
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
