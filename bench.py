@@ -410,6 +410,7 @@ def main():
             "waves_per_group": st["waves_per_group"], "sweeps_per_launch": st["sweeps_per_launch"],
             "launches": st["launches"], "task_tiling": {1: "dealt in turn", 2: "contiguous runs"}[st["task_tiling"]],
             "hipgraph": st["graph"],
+            "icm_growth": {0: "per-instance default", 1: "strided sites", 2: "compact tiles"}[st["icm_growth"]],
         },
         "kernel_attempts_per_s": kernel_rate,
         # where a step's time outside the kernel goes: library preparation (planning, allocation,

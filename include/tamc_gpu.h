@@ -224,12 +224,18 @@ typedef struct {
   int32_t autotune;          /* choose every setting left at 0 by timing the candidates on a 24-sweep
                                 prefix of the same batch (measurements off) and keeping the fastest;
                                 choices are cached per batch shape.  Candidates: 4, 8 and 16 waves per
-                                group x both task tilings, then one launch for all sweeps against
-                                four launches of a quarter each (captured in a hipGraph) */
+                                group x both task tilings, then both ICM growth orders, then one
+                                launch for all sweeps against four launches of a quarter each
+                                (captured in a hipGraph) */
   uint32_t task_tiling;      /* how a colour level's tasks (4-site RNG groups x plane words) are laid
                                 over a group's waves: 1 = dealt to the waves in turn, 2 = one
                                 contiguous run per wave; 0 = tuned with `autotune`, else 1.  Results
                                 do not depend on it */
+  uint32_t icm_growth;       /* how ICM clusters grow: 1 = each thread joins a fixed stride of sites
+                                per round, 2 = a wave closes a compact 64-site tile of the coupling
+                                graph at a time; 0 = tuned with `autotune`, else tiles for instances
+                                whose rows hold more than 4 couplings and strides otherwise.  The
+                                clusters do not depend on it */
 } tamc_batch_options;
 
 typedef struct {
@@ -248,6 +254,8 @@ typedef struct {
                                 compute unit was (one group per compute unit) */
   uint32_t task_tiling;      /* tiling used (1 dealt in turn, 2 contiguous runs) */
   int32_t graph;             /* 1 when the launches ran as a captured hipGraph */
+  uint32_t icm_growth;       /* ICM growth used (1 strided, 2 compact tiles, 0 the per-instance
+                                default) */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

@@ -337,11 +337,12 @@ def _batch_args(instances, params_list):
 
 
 def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False,
-                 autotune=False, init_states=None, task_tiling=0):
+                 autotune=False, init_states=None, task_tiling=0, icm_growth=0):
     """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path.
     init_states: optional list (None entries allowed) of [C][B][n] +-1 start states per solve.
     task_tiling: 1 = a level's tasks dealt to the waves in turn, 2 = one contiguous run per wave,
-    0 = tuned with `autotune` (else 1)."""
+    0 = tuned with `autotune` (else 1).  icm_growth: 1 = strided sites per thread, 2 = compact tiles,
+    0 = tuned with `autotune` (else tiles when the instance's rows hold more than 4 couplings)."""
     resolved, cps, hs = _batch_args(instances, params_list)
     outs = (C.POINTER(PtResult) * len(instances))()
     stats = BatchStats()
@@ -350,10 +351,12 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
     elif init_states is not None:
         keep = [None if s is None else np.ascontiguousarray(s, np.int8) for s in init_states]
         ptrs = (_i8p * len(instances))(*[_i8p() if s is None else ptr(s, _i8p) for s in keep])
-        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling)
+        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling,
+                           icm_growth)
         check(lib().tamc_pt_run_batch_init(len(instances), hs, cps, ptrs, C.byref(opt), outs, C.byref(stats)))
     else:
-        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling)
+        opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling,
+                           icm_growth)
         check(lib().tamc_pt_run_batch(len(instances), hs, cps, C.byref(opt), outs, C.byref(stats)))
     res = []
     try:
@@ -366,7 +369,8 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
               waves_per_group=stats.waves_per_group, sweeps_per_launch=stats.sweeps_per_launch,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
               icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms,
-              groups_busy_ms=stats.groups_busy_ms, task_tiling=stats.task_tiling, graph=bool(stats.graph))
+              groups_busy_ms=stats.groups_busy_ms, task_tiling=stats.task_tiling, graph=bool(stats.graph),
+              icm_growth=stats.icm_growth)
     return res, st
 
 

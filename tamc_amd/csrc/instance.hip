@@ -245,6 +245,49 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
       return nullptr;
     }
     D->ell_D = Dl;
+    if (Dl && inst->n < 65536u) {
+      // compact tiles of at most 64 sites for ICM cluster growth: balls of the coupling graph grown
+      // breadth-first from the lowest unassigned site; a ball that runs out of unassigned
+      // neighbours before 64 sites goes on from the next lowest unassigned site
+      const uint32_t n = inst->n;
+      std::vector<uint32_t> tiles, queue, mark(n, 0u);
+      std::vector<uint8_t> taken(n, 0u);
+      uint32_t next = 0, stamp = 0, fill = 0;
+      while (true) {
+        while (next < n && taken[next]) ++next;
+        if (next >= n) break;
+        ++stamp;
+        queue.clear();
+        queue.push_back(next);
+        mark[next] = stamp;
+        for (size_t h = 0; h < queue.size() && fill < 64u; ++h) {
+          const uint32_t i = queue[h];
+          taken[i] = 1u;
+          tiles.push_back(i);
+          ++fill;
+          for (uint32_t k = 0; k < Dl; ++k) {
+            const uint32_t j = ec[(size_t)i * Dl + k];
+            if (!taken[j] && mark[j] != stamp) {
+              mark[j] = stamp;
+              queue.push_back(j);
+            }
+          }
+        }
+        if (fill == 64u) fill = 0;  // the tile is full: the next ball starts a new one
+      }
+      while (tiles.size() % 64u) tiles.push_back(0xFFFFFFFFu);
+      // the 16-bit neighbour rows again in tile order, so that a wave loads its tile's rows with
+      // one coalesced access beside the site ids (padding positions hold zeros and are not read)
+      std::vector<uint16_t> trows(tiles.size() * Ds, 0u);
+      for (size_t k = 0; k < tiles.size(); ++k)
+        if (tiles[k] != 0xFFFFFFFFu)
+          memcpy(&trows[k * Ds], &ec16[(size_t)tiles[k] * Ds], Ds * 2);
+      if (!up(D->icm_tiles, tiles.data(), tiles.size() * 4) || !up(D->icm_trows, trows.data(), trows.size() * 2)) {
+        *st = fail(TAMC_E_DEVICE, "instance upload failed");
+        return nullptr;
+      }
+      D->icm_ntiles = (uint32_t)(tiles.size() / 64u);
+    }
     std::vector<float> sj(inst->n, 0.0f);
     for (uint32_t i = 0; i < inst->n; ++i)
       for (uint32_t e = inst->row_ptr[i]; e < inst->row_ptr[i + 1]; ++e)

@@ -672,6 +672,103 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
   if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs);
 }
 
+// ICM step (c) over compact tiles: the instance's sites are grouped into tiles of at most 64 that
+// are balls of the coupling graph (icm_tiles, padded with kNone), and a wave takes a tile at a
+// time, lane = site.  Within the tile the wave joins sites until nothing in the tile changes (or
+// kTileIters rounds), reading the cluster words its own lanes have just written, so a cluster
+// crosses a whole tile in one visit instead of one hop per round.  Every site belongs to one tile
+// and a tile to one wave per pass, so each cluster word has a single writer; cluster words only
+// gain bits, and the passes repeat until one changes nothing, which reaches the same fixpoint as
+// any other visiting order.
+constexpr uint32_t kTileIters = 24u;
+
+template <uint32_t S, uint32_t RW, bool kLds>
+__device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t D) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t nt = X.icm_ntiles;
+  // a tile's rows: the site-order copy in LDS when the lists live there, else the tile-order copy
+  // (one coalesced load, independent of the site id)
+  auto rows = [&](uint32_t pos, uint32_t i, uint64_t &w0, uint64_t &w1) {
+    if (M.nbr_lds) {
+      const uint32_t ii = i < n ? i : 0u;
+      w0 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * ii];
+      if (RW == 2u) w1 = reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * ii + 1u];
+    } else {
+      w0 = reinterpret_cast<const TAMC_AS4 uint64_t *>(c4(X.icm_trows))[RW * pos];
+      if (RW == 2u) w1 = reinterpret_cast<const TAMC_AS4 uint64_t *>(c4(X.icm_trows))[RW * pos + 1u];
+    }
+  };
+  if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
+  __syncthreads();
+  uint32_t label_pairs = 0u;
+  for (uint32_t pass = 0;; ++pass) {
+    if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+    uint32_t gain = 0u;
+    // the next tile's site ids and rows are requested before the current tile is worked on
+    uint32_t tl = c.wave, i = kNone;
+    uint64_t w0 = 0ull, w1 = 0ull;
+    if (tl < nt) {
+      i = c4(X.icm_tiles)[tl * 64u + c.lane];
+      rows(tl * 64u + c.lane, i, w0, w1);
+    }
+    while (tl < nt) {
+      const uint32_t tn = tl + c.W;
+      uint32_t ni = kNone;
+      uint64_t n0 = 0ull, n1 = 0ull;
+      if (tn < nt) {
+        ni = c4(X.icm_tiles)[tn * 64u + c.lane];
+        rows(tn * 64u + c.lane, ni, n0, n1);
+      }
+      const bool in = i < n;
+      uint32_t q = 0u, cw = 0u;
+      if (in) {
+        q = M.qw[i];
+        cw = M.cw[i];
+      }
+      if (ballot(cw != q)) {  // else every site of the tile joined, or none can
+#pragma unroll 1
+        for (uint32_t it = 0; it < kTileIters; ++it) {
+          uint32_t x = 0u;
+          if (cw != q) {
+            x = M.cw[w0 & 0xFFFFu] | M.cw[(w0 >> 16) & 0xFFFFu] | M.cw[(w0 >> 32) & 0xFFFFu];
+            if (D >= 4u) x |= M.cw[w0 >> 48];
+            if (RW == 2u) {
+              x |= M.cw[w1 & 0xFFFFu];
+              if (D >= 6u) x |= M.cw[(w1 >> 16) & 0xFFFFu];
+              if (D >= 7u) x |= M.cw[(w1 >> 32) & 0xFFFFu];
+              if (D == 8u) x |= M.cw[w1 >> 48];
+            }
+          }
+          const uint32_t nc = cw | (q & x);
+          const bool ch = nc != cw;
+          if (ch) {
+            M.cw[i] = nc;
+            gain |= nc ^ cw;
+            cw = nc;
+          }
+          if (!ballot(ch)) break;
+        }
+      }
+      tl = tn;
+      i = ni;
+      w0 = n0;
+      w1 = n1;
+    }
+    const uint32_t g = wave_or(gain);
+    uint32_t *slot = c.misc + 8u + pass % 3u;
+    if (c.lane == 0u && g) atomicOr(slot, g);
+    __syncthreads();
+    const uint32_t grew = *slot;
+    if (c.tid == 0u) c.misc[8u + (pass + 2u) % 3u] = 0u;
+    if (grew == 0u) break;
+    if (pass + 1u >= kLabelAfterPasses && (uint32_t)__popc(grew) <= kMaxLabelPairs) {
+      label_pairs = grew;
+      break;
+    }
+  }
+  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs);
+}
+
 // ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster
 // when its q bit p is set and a neighbour is in it.  Passes update in place until a pass changes
 // nothing; each pass makes several rounds over the sites between barriers (four when the sites
@@ -682,7 +779,18 @@ template <bool kLds>
 __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t n = c.n, D = c.nbr ? X.ell_D : 0u;
-  if (D != 0u && D <= 4u && n <= 4u * c.nthreads) {
+  if ((c.flags & kFlagIcmTiles) && X.icm_tiles != nullptr && D != 0u && n <= 16u * c.nthreads) {
+    const uint32_t S = n <= 4u * c.nthreads ? 4u : (n <= 8u * c.nthreads ? 8u : 16u);
+    if (D <= 4u) {
+      if (S == 4u) icm_grow_tiles<4, 1>(c, M, n, D);
+      else if (S == 8u) icm_grow_tiles<8, 1>(c, M, n, D);
+      else icm_grow_tiles<16, 1>(c, M, n, D);
+    } else {
+      if (S == 4u) icm_grow_tiles<4, 2>(c, M, n, D);
+      else if (S == 8u) icm_grow_tiles<8, 2>(c, M, n, D);
+      else icm_grow_tiles<16, 2>(c, M, n, D);
+    }
+  } else if (D != 0u && D <= 4u && n <= 4u * c.nthreads) {
     icm_grow_registers<4, 1>(c, M, n, D);
   } else if (D != 0u && D <= 4u && n <= 8u * c.nthreads) {
     icm_grow_registers<8, 1>(c, M, n, D);

@@ -177,6 +177,9 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.ell_col16 = S.n < 65536u ? (const uint16_t *)D->ell_col16.p : nullptr;
     X.ell_J = (const float *)D->ell_J.p;
     X.ell_D = D->ell_D;
+    X.icm_tiles = X.ell_col16 ? (const uint32_t *)D->icm_tiles.p : nullptr;
+    X.icm_ntiles = X.icm_tiles ? D->icm_ntiles : 0u;
+    X.icm_trows = X.icm_tiles ? (const uint16_t *)D->icm_trows.p : nullptr;
     X.self_J = D->has_self ? (const float *)D->self_J.p : nullptr;
     X.offset = S.inst->offset;
     X.n = S.n;
@@ -198,7 +201,7 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.P = g.P;
     X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
               (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
-              (g.icm_global ? kFlagIcmGlobal : 0u);
+              (g.icm_global ? kFlagIcmGlobal : 0u) | (X.icm_tiles && X.ell_D > 4u ? kFlagIcmTiles : 0u);
     X.T = S.T;
     X.meas_init = S.meas_init;
     X.sample = S.P.sample_enabled ? S.P.sample : 1u;
@@ -242,7 +245,8 @@ struct Launch {
   }
 };
 
-static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, uint32_t tiling, Launch &Lc) {
+static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, uint32_t tiling, uint32_t growth,
+                                  Launch &Lc) {
   struct Item {
     SolvePlan *S;
     GroupPlan *g;
@@ -261,6 +265,8 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
   for (auto &it : items) {
     Lc.order.push_back(it.g->g);
     if (tiling == 2u) Lc.order.back().flags |= kFlagTileBlocks;
+    if (growth == 1u) Lc.order.back().flags &= ~kFlagIcmTiles;
+    if (growth == 2u && Lc.order.back().icm_tiles) Lc.order.back().flags |= kFlagIcmTiles;
     if (any_init) {
       if (it.g->init_bits.empty()) {
         Lc.init_ptrs.push_back(nullptr);
@@ -462,6 +468,8 @@ struct RunOptions {
   uint32_t waves = 8;
   uint32_t sweeps_per_launch = 0;  // 0 = all
   uint32_t tiling = 1;             // 1 tasks dealt to the waves in turn, 2 contiguous runs
+  uint32_t growth = 0;             // ICM cluster growth: 1 strided sites per thread, 2 compact tiles,
+                                   // 0 each group's default (tiles for rows wider than 4 entries)
   bool use_graph = false;
   std::chrono::steady_clock::time_point t_entry = std::chrono::steady_clock::now();  // for prep_ms
 };
@@ -474,7 +482,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (st) return st;
   if (patch) patch(solves);
   Launch Lc;
-  if ((st = prepare_launch(solves, opt.waves, opt.tiling, Lc))) return st;
+  if ((st = prepare_launch(solves, opt.waves, opt.tiling, opt.growth, Lc))) return st;
   const uint32_t ng = (uint32_t)Lc.order.size();
   hipStream_t s;
   TAMC_HIP(hipStreamCreate(&s));
@@ -580,6 +588,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
     stats->launches = launches;
     stats->groups = ng;
     stats->task_tiling = opt.tiling;
+    stats->icm_growth = opt.growth;
     stats->graph = opt.use_graph ? 1 : 0;
     stats->prep_ms = std::chrono::duration<double, std::milli>(w0 - opt.t_entry).count();
     stats->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w1).count();
@@ -688,7 +697,7 @@ struct TuneKey {
   }
 };
 struct TuneChoice {
-  uint32_t waves = 16, tiling = 1;
+  uint32_t waves = 16, tiling = 1, growth = 0;
   bool quarters = false;  // a quarter of the sweeps per launch, launches captured in a hipGraph
 };
 std::mutex g_tune_mu;
@@ -697,7 +706,8 @@ constexpr uint32_t kTunePrefix = 24u;
 }  // namespace
 
 static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
-                                  uint32_t fixed_waves, uint32_t fixed_tiling, TuneChoice *out) {
+                                  uint32_t fixed_waves, uint32_t fixed_tiling, uint32_t fixed_growth,
+                                  TuneChoice *out) {
   TuneKey key{count, 0, 0, 0, 0, 0};
   for (uint32_t k = 0; k < count; ++k) {
     key.order |= 1u << params[k].sweep_order;
@@ -706,7 +716,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
     key.B = std::max(key.B, params[k].num_betas);
     key.C = std::max(key.C, params[k].num_replica_chains);
   }
-  key.order |= (fixed_waves << 8) | (fixed_tiling << 16);
+  key.order |= (fixed_waves << 8) | (fixed_tiling << 16) | (fixed_growth << 24);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -749,6 +759,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
       RunOptions opt;
       opt.waves = w;
       opt.tiling = tl;
+      opt.growth = fixed_growth;
       double ms;
       const tamc_status st = time_one(opt, &ms);
       if (st) return st;
@@ -764,10 +775,26 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
     best.waves = fixed_waves;
     best.tiling = fixed_tiling ? fixed_tiling : 1u;
   }
+  best.growth = fixed_growth;
+  for (uint32_t gr : {1u, 2u}) {
+    if (!any || fixed_growth) break;
+    RunOptions opt;
+    opt.waves = best.waves;
+    opt.tiling = best.tiling;
+    opt.growth = gr;
+    double ms;
+    const tamc_status st = time_one(opt, &ms);
+    if (st) return st;
+    if (ms >= 0.0 && ms < best_ms) {
+      best_ms = ms;
+      best.growth = gr;
+    }
+  }
   if (any) {
     RunOptions opt;
     opt.waves = best.waves;
     opt.tiling = best.tiling;
+    opt.growth = best.growth;
     opt.sweeps_per_launch = (kTunePrefix + 3u) / 4u;
     opt.use_graph = true;
     double ms;
@@ -803,15 +830,20 @@ tamc_status tamc_pt_run_batch_init(uint32_t count, const tamc_instance *const *i
   if (opts && opts->waves_per_group) w = opts->waves_per_group;
   if (w < 1 || w > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
   if (opts && opts->task_tiling > 2u) return fail(TAMC_E_INVALID_ARG, "task_tiling must be 0, 1 or 2");
+  if (opts && opts->icm_growth > 2u) return fail(TAMC_E_INVALID_ARG, "icm_growth must be 0, 1 or 2");
   opt.waves = w;
   opt.tiling = (opts && opts->task_tiling) ? opts->task_tiling : 1u;
+  opt.growth = opts ? opts->icm_growth : 0u;
   opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
   opt.use_graph = opts && opts->use_graph;
-  if (opts && opts->autotune && count && (!opts->waves_per_group || !opts->task_tiling || !opts->sweeps_per_launch)) {
+  if (opts && opts->autotune && count &&
+      (!opts->waves_per_group || !opts->task_tiling || !opts->icm_growth || !opts->sweeps_per_launch)) {
     TuneChoice tc;
-    if ((st = autotune_batch(count, insts, params, opts->waves_per_group, opts->task_tiling, &tc))) return st;
+    if ((st = autotune_batch(count, insts, params, opts->waves_per_group, opts->task_tiling, opts->icm_growth, &tc)))
+      return st;
     opt.waves = tc.waves;
     opt.tiling = tc.tiling;
+    opt.growth = tc.growth;
     if (!opts->sweeps_per_launch && tc.quarters) {
       uint32_t maxT = 0;
       for (uint32_t k = 0; k < count; ++k) maxT = std::max(maxT, params[k].num_sweeps);

@@ -31,6 +31,7 @@ enum : uint32_t {
   kFlagIcmOnly = 32u,        // run only the ICM step with seeds given in icm_k (tamc_houdayer)
   kFlagTileBlocks = 64u,     // sweep tiling: each wave takes one contiguous run of a level's tasks
                              // (else the tasks are dealt to the waves in turn)
+  kFlagIcmTiles = 128u,      // ICM cluster growth over compact tiles of the coupling graph
 };
 
 struct PtGroup {
@@ -85,7 +86,9 @@ struct PtGroup {
   const uint32_t *ell_col;   // [n][ell_D], padding entries point at the row's own site
   const uint16_t *ell_col16; // 16-bit neighbour lists [n][nbr_stride(ell_D)] (n < 65536), for ICM
   const float *ell_J;        // [n][ell_D], padding entries are +0.0
-  uint32_t ell_D, pad2;
+  uint32_t ell_D, icm_ntiles;
+  const uint32_t *icm_tiles; // [icm_ntiles][64] sites of each compact tile (kNone pads), for ICM
+  const uint16_t *icm_trows; // [icm_ntiles * 64][nbr_stride(ell_D)] their neighbour rows, tile order
   // ---- QUBO self entries (src/ising.rs:308-310): J_ii per site, null when the instance has none.
   // Their flips shift the reference's tracked energy by -2 J_ii each until energy_ref is recomputed
   // after an ICM move; `drift` carries that shift (2^-32 fixed point) per replica.

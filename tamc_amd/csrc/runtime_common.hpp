@@ -104,6 +104,9 @@ struct DeviceInstance {
   // row entries -- eight neighbour byte offsets in a wave's spin planes (8 * col), then the eight
   // couplings' bits; the last block of a row is padded with the site itself and +0.0
   DevBuf sa_info, sa_blocks;
+  DevBuf icm_tiles;           // [icm_ntiles][64] compact tiles of the coupling graph for ICM growth
+  DevBuf icm_trows;           // [icm_ntiles * 64][nbr_stride(ell_D)] their 16-bit rows, in tile order
+  uint32_t icm_ntiles = 0;
   bool has_self = false;
   uint32_t ell_D = 0;
   std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by 2 * (words per site G) + order

@@ -7,8 +7,8 @@
       vectors: every output bit-exact (the oracle's production mode restates the device's exact
       tracked-energy accounting).
   c3  the benchmark's own 256 random 3-regular graphs (N ~ U{200..4000}, J~N(0,1), h~N(0,0.1)) in
-      one autotuned launch and in 16-wave launches with either task tiling, 16 sweeps with ICM
-      from sweep 0: every instance against the oracle.
+      one autotuned launch and in 16-wave launches with either task tiling and either ICM growth
+      order, 16 sweeps with ICM from sweep 0: every instance against the oracle.
   c4  64 QUBOs N=2048 at 5 % (Q ~ U{-100..100}, diagonal included), 1024 restarts, one batched SA
       launch over 8 sweeps of a linear 0.1 -> 10 schedule: three instances state by state.
   c5  a seeded mini-stream from the serving distribution plus its largest shapes (2D L=128,
@@ -119,9 +119,9 @@ def test_c3_bench_batch_autotuned_and_16_waves(gpu):
     tuned, tst = api.run_pt_batch(insts, params, autotune=True)
     assert tst["waves_per_group"] in (4, 8, 16) and tst["launches"] in (1, 4) and tst["groups"] == 256
     assert tst["task_tiling"] in (1, 2)
-    wide, wst = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=1)
+    wide, wst = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=1, icm_growth=1)
     assert wst["waves_per_group"] == 16
-    runs, _ = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=2)
+    runs, _ = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=2, icm_growth=2)
     assert max(x.n for x in insts) > 3900  # the largest group runs the LDS-resident growth path
     oracles = _oracle_many(ws, params, betas, lo)
     for k, o in enumerate(oracles):

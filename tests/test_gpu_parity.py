@@ -176,14 +176,18 @@ def test_icm_growth_every_sites_per_thread(gpu):
                                       lo_num_beta=4, icm=True, num_replica_chains=C, threads=1, sample=2,
                                       sample_states=3, sample_limiting=2, seed=900 + k))
     assert [g.n for g in insts] == [144, 400, 900, 216, 343, 1000, 1156, 1331]
-    batch, st = api.run_pt_batch(insts, params, waves_per_group=1)
-    assert st["waves_per_group"] == 1
+    # strided sites per thread, and compact 64-site tiles closed one wave at a time
+    strided, st = api.run_pt_batch(insts, params, waves_per_group=1, icm_growth=1)
+    assert st["waves_per_group"] == 1 and st["icm_growth"] == 1
+    tiles, st2 = api.run_pt_batch(insts, params, waves_per_group=1, icm_growth=2)
+    assert st2["icm_growth"] == 2
     for k, (oi, p) in enumerate(zip(oracles, params)):
         runner = api.PtIcmRunner(insts[k], p)
         o = O.pt_run(oi, runner.betas, T, num_chains=C, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=True,
                      threaded=False, sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
         assert o["error"] == 0
-        assert_same(batch[k], o, exact_energy=True)
+        assert_same(strided[k], o, exact_energy=True)
+        assert_same(tiles[k], o, exact_energy=True)
 
 
 def test_icm_staged_growth_over_memory(gpu):
@@ -290,9 +294,11 @@ def test_batch_equals_single_and_plain(gpu):
     runs, rst = api.run_pt_batch(insts, params, waves_per_group=16, task_tiling=2, sweeps_per_launch=10,
                                  use_graph=True)
     assert rst["task_tiling"] == 2 and rst["launches"] == 4 and rst["graph"]
+    strided, sst = api.run_pt_batch(insts, params, icm_growth=1)
+    assert sst["icm_growth"] == 1
     for k in range(len(insts)):
         single = api.PtIcmRunner(insts[k], params[k]).run()
-        for other in (plain[k], chunked[k], tuned[k], runs[k], single):
+        for other in (plain[k], chunked[k], tuned[k], runs[k], strided[k], single):
             for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
                 assert np.array_equal(batch[k][key], other[key]), key
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
