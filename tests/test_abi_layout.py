@@ -98,7 +98,8 @@ def test_rust_binding_in_integration_notes_matches_c(c_layout):
     structs = dict(re.findall(r"pub struct (\w+) \{[ \t]*(?://[^\n]*)?\n(.*?)\n\}", text, re.S))
     checked = 0
     for rname, cname in [("TamcPtParams", "tamc_pt_params"), ("TamcSaParams", "tamc_sa_params"),
-                         ("TamcInstanceDesc", "tamc_instance_desc")]:
+                         ("TamcInstanceDesc", "tamc_instance_desc"), ("TamcPtState", "tamc_pt_state"),
+                         ("TamcBatchOptions", "tamc_batch_options")]:
         if rname not in structs:
             continue
         body = re.sub(r"//[^\n]*", "", structs[rname])
