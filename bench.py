@@ -201,7 +201,8 @@ def pt_plain_cases(with_cpu, cores):
         _, st = api.run_pt_batch([inst], [p], plain=True)
         dt = time.perf_counter() - t0
         rec = {"value": st["attempts"] / dt, "unit": "spin-flip attempts/s", "attempts": int(st["attempts"]),
-               "ms": dt * 1e3, "kernel_ms": st["device_ms"], "n": n, "work_groups": st["groups"],
+               "ms": dt * 1e3, "kernel_ms": st["device_ms"], "n": n, "work_groups": st["work_groups"],
+               "teams": st["teams"],
                "mean_cu_busy": st["groups_busy_ms"] / (CUS * st["device_ms"]) if st["device_ms"] else None}
         if with_cpu:
             import oracle as O
@@ -411,6 +412,7 @@ def main():
             "launches": st["launches"], "task_tiling": {1: "dealt in turn", 2: "contiguous runs"}[st["task_tiling"]],
             "hipgraph": st["graph"],
             "icm_growth": {0: "per-instance default", 1: "strided sites", 2: "compact tiles"}[st["icm_growth"]],
+            "teams": st["teams"], "work_groups": st["work_groups"],
         },
         "kernel_attempts_per_s": kernel_rate,
         # where a step's time outside the kernel goes: library preparation (planning, allocation,

@@ -224,9 +224,9 @@ typedef struct {
   int32_t autotune;          /* choose every setting left at 0 by timing the candidates on a 24-sweep
                                 prefix of the same batch (measurements off) and keeping the fastest;
                                 choices are cached per batch shape.  Candidates: 4, 8 and 16 waves per
-                                group x both task tilings, then both ICM growth orders, then one
-                                launch for all sweeps against four launches of a quarter each
-                                (captured in a hipGraph) */
+                                group x both task tilings, then both ICM growth orders, then teams
+                                against none, then one launch for all sweeps against four launches
+                                of a quarter each (captured in a hipGraph) */
   uint32_t task_tiling;      /* how a colour level's tasks (4-site RNG groups x plane words) are laid
                                 over a group's waves: 1 = dealt to the waves in turn, 2 = one
                                 contiguous run per wave; 0 = tuned with `autotune`, else 1.  Results
@@ -236,6 +236,12 @@ typedef struct {
                                 graph at a time; 0 = tuned with `autotune`, else tiles for instances
                                 whose rows hold more than 4 couplings and strides otherwise.  The
                                 clusters do not depend on it */
+  uint32_t teams;            /* a large group's Metropolis sweep shared by a team of work groups of
+                                one cooperative launch (small groups then share work groups, one
+                                after another): 0 = where a cost model predicts a shorter launch
+                                (with `autotune`: kept only if it times faster), 1 = never, K >= 2
+                                = every eligible group as a team of K (at most 16).  Results do not
+                                depend on it */
 } tamc_batch_options;
 
 typedef struct {
@@ -256,6 +262,8 @@ typedef struct {
   int32_t graph;             /* 1 when the launches ran as a captured hipGraph */
   uint32_t icm_growth;       /* ICM growth used (1 strided, 2 compact tiles, 0 the per-instance
                                 default) */
+  uint32_t teams;            /* groups that ran as teams of work groups */
+  uint32_t work_groups;      /* work groups of the sweep launch */
 } tamc_batch_stats;
 
 /* Many independent PT-ICM solves (one per instance) in one persistent kernel; each result is

@@ -71,7 +71,7 @@ class PtResult(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("waves_per_group", C.c_uint32), ("sweeps_per_launch", C.c_uint32), ("use_graph", C.c_int32),
                 ("autotune", C.c_int32), ("task_tiling", C.c_uint32),
-                ("icm_growth", C.c_uint32)]
+                ("icm_growth", C.c_uint32), ("teams", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -79,7 +79,8 @@ class BatchStats(C.Structure):
                 ("waves_per_group", C.c_uint32), ("sweeps_per_launch", C.c_uint32), ("launches", C.c_uint32),
                 ("groups", C.c_uint32), ("phase_ms", C.c_double * 7), ("icm_passes", C.c_double),
                 ("prep_ms", C.c_double), ("post_ms", C.c_double), ("groups_busy_ms", C.c_double),
-                ("task_tiling", C.c_uint32), ("graph", C.c_int32), ("icm_growth", C.c_uint32)]
+                ("task_tiling", C.c_uint32), ("graph", C.c_int32), ("icm_growth", C.c_uint32),
+                ("teams", C.c_uint32), ("work_groups", C.c_uint32)]
 
 
 class SaParams(C.Structure):

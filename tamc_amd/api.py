@@ -337,12 +337,14 @@ def _batch_args(instances, params_list):
 
 
 def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0, use_graph=False, plain=False,
-                 autotune=False, init_states=None, task_tiling=0, icm_growth=0):
+                 autotune=False, init_states=None, task_tiling=0, icm_growth=0, teams=0):
     """Many PT-ICM solves: batched persistent kernel (default) or the plain one-instance-per-launch path.
     init_states: optional list (None entries allowed) of [C][B][n] +-1 start states per solve.
     task_tiling: 1 = a level's tasks dealt to the waves in turn, 2 = one contiguous run per wave,
     0 = tuned with `autotune` (else 1).  icm_growth: 1 = strided sites per thread, 2 = compact tiles,
-    0 = tuned with `autotune` (else tiles when the instance's rows hold more than 4 couplings)."""
+    0 = tuned with `autotune` (else tiles when the instance's rows hold more than 4 couplings).
+    teams: 0 = large groups' sweeps shared by teams of work groups where a cost model (and, with
+    `autotune`, a timing) says it pays, 1 = never, K >= 2 = every eligible group as a team of K."""
     resolved, cps, hs = _batch_args(instances, params_list)
     outs = (C.POINTER(PtResult) * len(instances))()
     stats = BatchStats()
@@ -352,11 +354,11 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
         keep = [None if s is None else np.ascontiguousarray(s, np.int8) for s in init_states]
         ptrs = (_i8p * len(instances))(*[_i8p() if s is None else ptr(s, _i8p) for s in keep])
         opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling,
-                           icm_growth)
+                           icm_growth, teams)
         check(lib().tamc_pt_run_batch_init(len(instances), hs, cps, ptrs, C.byref(opt), outs, C.byref(stats)))
     else:
         opt = BatchOptions(waves_per_group, sweeps_per_launch, int(use_graph), int(autotune), task_tiling,
-                           icm_growth)
+                           icm_growth, teams)
         check(lib().tamc_pt_run_batch(len(instances), hs, cps, C.byref(opt), outs, C.byref(stats)))
     res = []
     try:
@@ -370,7 +372,7 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
               launches=stats.launches, groups=stats.groups, phase_ms=list(stats.phase_ms),
               icm_passes=stats.icm_passes, prep_ms=stats.prep_ms, post_ms=stats.post_ms,
               groups_busy_ms=stats.groups_busy_ms, task_tiling=stats.task_tiling, graph=bool(stats.graph),
-              icm_growth=stats.icm_growth)
+              icm_growth=stats.icm_growth, teams=stats.teams, work_groups=stats.work_groups)
     return res, st
 
 

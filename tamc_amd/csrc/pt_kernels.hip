@@ -170,6 +170,7 @@ struct Ctx {
   unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [k][v] = replicas of
                               // the pairs 4k + j with bit j of v set (one plane word per site)
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
+  uint32_t members, member;
   bool icm_lds;
 };
 
@@ -180,6 +181,47 @@ __device__ __forceinline__ void lap(const Ctx &c, int ph) {
     c.ticks[ph] += now - c.ticks[kPhaseSlots];
     c.ticks[kPhaseSlots] = now;
   }
+}
+
+// Meeting point of a team (the work groups that share a group's sweep).  Each member's global
+// stores before the call are visible to every member after it.  Waits are bounded: a member that
+// waits longer than kTeamSpinLimit polls raises the team's abort flag, and once the flag is up no
+// member waits again, so the launch always ends (the host then reports the run as failed).  The
+// member's count of meetings is kept in misc[15] (misc[12..14] are the labelling's round flags).
+constexpr uint32_t kTeamSpinLimit = 1u << 20;
+
+// The words the members exchange are written and read with agent-scope atomic stores and loads,
+// which are coherent across the XCDs' L2 caches without a fence: a fence at agent scope would
+// write back and invalidate the whole L2 of the XCD, which every other group's sweep records
+// share.  Each wave waits until its stores have completed before the work group arrives.
+__device__ __forceinline__ void box_store(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long box_load(const unsigned long long *p) {
+  return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void team_sync(const Ctx &c) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's box stores have completed
+  __syncthreads();
+  if (c.tid == 0u) {
+    uint32_t *ctr = X.team_ctr;
+    const uint32_t e = c.misc[15] + 1u;
+    c.misc[15] = e;
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t target = e * c.members;
+    uint32_t spins = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(ctr + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (++spins > kTeamSpinLimit) {
+        __hip_atomic_store(ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
 }
 
 // Spin bits of one site for replica indices chosen at run time.  With one 64-bit plane word per
@@ -1191,7 +1233,7 @@ __device__ __forceinline__ float site_field(const Ctx &c, uint32_t i, uint32_t g
 // with the hardware exp2 (accept when u is below the estimate by more than the error band, reject
 // when above), and, only when some lane of the wave fell inside the band, the exact test.
 
-template <int D>
+template <int D, bool kTeam>
 __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t G = c.G;
@@ -1204,18 +1246,27 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   const bool self = X.self_J != nullptr;
   uint32_t a = c4(X.lvl_off)[0];
   const bool blocks = (c.flags & kFlagTileBlocks) != 0u;
+  const uint32_t K = kTeam ? c.members : 1u;
   for (uint32_t l = 0; l < nl; ++l) {
     const uint32_t b = c4(X.lvl_off)[l + 1];
-    // tiling of the level's tasks [a, b) over the waves: dealt in turn, or one contiguous run each
+    // tiling of the level's tasks [a, b) over the waves: dealt in turn, or one contiguous run each.
+    // In a team the member takes the RNG groups g == member (mod members) of the level, dealt in
+    // turn: index j stands for word j % G of the level's (j / G)-th such group.
+    const uint32_t ga = a / G, gb = b / G;
+    const uint32_t gm = ga + (c.member + K - ga % K) % K;
     uint32_t k = a + c.wave, kend = b, kstep = c.W;
-    if (blocks) {
+    if (K > 1u) {
+      k = c.wave;
+      kend = gm < gb ? (gb - gm + K - 1u) / K * G : 0u;
+    } else if (blocks) {
       const uint32_t per = (b - a + c.W - 1u) / c.W;
       k = a + c.wave * per;
       kend = (k + per < b) ? k + per : b;
       kstep = 1u;
     }
     for (; k < kend; k += kstep) {
-      const uint32_t grp = (G == 1u) ? k : k / G, g = k - grp * G;
+      const uint32_t kk = (K > 1u) ? (gm + (k / G) * K) * G + k % G : k;
+      const uint32_t grp = (G == 1u) ? kk : kk / G, g = kk - grp * G;
       const uint32_t r = g * 64u + c.lane;
       const bool valid = r < c.R;
       const float beta = valid ? c.beta_of[r] : 0.0f;
@@ -1306,6 +1357,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         // the site's new plane word: the lanes' old bits with the accepted ones flipped
         const uint64_t nw = ballot(bi[s] != 0u) ^ accm;
         if (c.lane == 0u) *reinterpret_cast<uint64_t *>(c.sp + (i * G + g) * 2u) = nw;
+        if (K > 1u && c.lane == 0u) box_store(X.team_box + (size_t)i * G + g, nw);
         const bool acc_ok = __builtin_amdgcn_inverse_ballot_w64(accm);
         if (acc_ok) a_g += dd[s];
         if (self) {
@@ -1316,10 +1368,40 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       // the group's accepted dE, summed in group order, enter the fixed-point total
       atomicAdd(acc + g * 64u, (unsigned long long)to_fix(a_g));
     }
+    if (K > 1u) {
+      // take the other members' new spin words of this level
+      team_sync(c);
+      const uint32_t items = (gb - ga) * 4u * G;
+      for (uint32_t it = c.tid; it < items; it += c.nthreads) {
+        const uint32_t q = ga + it / (4u * G);
+        if (q % K == c.member) continue;
+        const uint32_t site = c4(X.grp)[q * 4u + (it / G) % 4u], g = it % G;
+        if (site != kNone)
+          *reinterpret_cast<uint64_t *>(c.sp + (site * G + g) * 2u) = box_load(X.team_box + (size_t)site * G + g);
+      }
+    }
     a = b;
     __syncthreads();
   }
   __syncthreads();
+  if (K > 1u) {
+    // every member adds every member's share, in member order: the same energies everywhere
+    unsigned long long *sums = X.team_box + (size_t)c.n * G;
+    for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
+      const uint32_t g = r >> 6, l = r & 63u;
+      long long s = 0;
+      for (uint32_t w = 0; w < c.W; ++w) s += c.partial[(w * G + g) * 64u + l];
+      box_store(sums + (size_t)c.member * c.R + r, (unsigned long long)s);
+    }
+    team_sync(c);
+    for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
+      long long s = 0;
+      for (uint32_t m = 0; m < K; ++m) s += (long long)box_load(sums + (size_t)m * c.R + r);
+      c.energy[r] = add_fix(c.energy[r], s);
+    }
+    __syncthreads();
+    return;
+  }
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
     const uint32_t g = r >> 6, l = r & 63u;
     long long s = 0;
@@ -1329,13 +1411,25 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   __syncthreads();
 }
 
+// the team variant is a separate instantiation, so that groups run by one work group keep the
+// registers the team bookkeeping would take
 __device__ __forceinline__ void sweep_phase(Ctx &c, uint32_t t) {
+  if (c.members > 1u) {
+    switch (c.X->ell_D) {
+      case 3u: sweep_phase_d<3, true>(c, t); break;
+      case 4u: sweep_phase_d<4, true>(c, t); break;
+      case 6u: sweep_phase_d<6, true>(c, t); break;
+      case 8u: sweep_phase_d<8, true>(c, t); break;
+      default: sweep_phase_d<0, true>(c, t);
+    }
+    return;
+  }
   switch (c.X->ell_D) {
-    case 3u: sweep_phase_d<3>(c, t); break;
-    case 4u: sweep_phase_d<4>(c, t); break;
-    case 6u: sweep_phase_d<6>(c, t); break;
-    case 8u: sweep_phase_d<8>(c, t); break;
-    default: sweep_phase_d<0>(c, t);
+    case 3u: sweep_phase_d<3, false>(c, t); break;
+    case 4u: sweep_phase_d<4, false>(c, t); break;
+    case 6u: sweep_phase_d<6, false>(c, t); break;
+    case 8u: sweep_phase_d<8, false>(c, t); break;
+    default: sweep_phase_d<0, false>(c, t);
   }
 }
 
@@ -1459,9 +1553,9 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
 // pointer passed through an empty asm statement: that keeps the compiler from holding dozens of
 // loop-invariant scalars live across all phases, which it would otherwise spill to VGPR lanes and
 // read back with v_readlane inside the hot loops.
-__device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds, uint32_t lds_bytes) {
+__device__ __forceinline__ Ctx make_ctx(const PtGroup *group, uint32_t icm_lds, uint32_t lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const TAMC_AS4 PtGroup &X = *c4(groups + blockIdx.x);
+  const TAMC_AS4 PtGroup &X = *c4(group);
   Ctx c;
   c.X = &X;
   c.n = X.n;
@@ -1473,6 +1567,8 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds,
   c.lo = X.lo;
   c.flags = X.flags;
   c.nblk = X.nblk;
+  c.members = X.members;
+  c.member = X.member;
   c.W = blockDim.x / 64u;
   c.tid = threadIdx.x;
   c.nthreads = blockDim.x;
@@ -1527,16 +1623,14 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *groups, uint32_t icm_lds,
   return c;
 }
 
-__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
-                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
-                                                        unsigned long long *launch_ticks) {
-  if (ranges) {
-    t0 = c4(ranges)[2u * blockIdx.x];
-    t1 = c4(ranges)[2u * blockIdx.x + 1u];
-  }
-  const unsigned long long launch_start = wall_clock64();
-  Ctx c = make_ctx(groups, icm_lds, lds_bytes);
+// One group (or one team member's copy of it): load the persistent state into LDS, run sweeps
+// [t0, t1), store the state back.
+__device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uint32_t t1, uint32_t icm_lds,
+                                          uint32_t lds_bytes, unsigned long long launch_start,
+                                          unsigned long long *launch_ticks) {
+  Ctx c = make_ctx(group, icm_lds, lds_bytes);
   const TAMC_AS4 PtGroup &X = *c.X;
+  const bool lead = c.member == 0u;  // the member that writes outputs and the final state
   if (c.P > 0 && c.nbr_in_lds) {
     uint16_t *nb = const_cast<uint16_t *>(c.nbr);
     for (uint32_t k = c.tid; k < c.n * nbr_stride(X.ell_D); k += c.nthreads) nb[k] = c4(X.ell_col16)[k];
@@ -1544,6 +1638,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   if (c.tid == 0) {
     c.moves[0] = 0u;
     for (int k = 0; k <= (int)kPhaseSlots; ++k) c.ticks[k] = 0ull;
+    c.misc[15] = c.members > 1u ? X.team_ctr[3u + c.member] : 0u;  // team meetings so far
   }
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
@@ -1568,7 +1663,7 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   // one iteration per ICM chunk of at most kMaxPairs slots (a pair's moves depend on its own slot
   // only); the last chunk's iteration goes on with the rest of the sweep
   for (uint32_t t = t0, ck = 0; t < tend;) {
-    const PtGroup *gp = groups;
+    const PtGroup *gp = group;
     uint32_t il = icm_lds, lb = lds_bytes;
     asm volatile("" : "+s"(gp), "+s"(il), "+s"(lb));
     Ctx cs = make_ctx(gp, il, lb);
@@ -1587,11 +1682,13 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
     lap(cs, kPhExchange);
     sweep_phase(cs, t);
     lap(cs, kPhSweep);
-    measure_phase(cs, t);
+    if (cs.member == 0u) measure_phase(cs, t);
     lap(cs, kPhMeasure);
     ++t;
   }
   __syncthreads();
+  if (c.tid == 0 && c.members > 1u) X.team_ctr[3u + c.member] = c.misc[15];
+  if (!lead) return;
   if (c.tid == 0 && X.phase_ticks)
     for (int k = 0; k < (int)kPhaseSlots; ++k) X.phase_ticks[k] += c.ticks[k];
   __syncthreads();
@@ -1613,9 +1710,44 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   if (c.tid < c.nch) X.rt[c.tid] = c.rt[c.tid];
 }
 
+// Work group b runs the groups jobs[b] .. jobs[b + 1] - 1 one after another (without a job table,
+// group b alone).  The members of a team are in different work groups of one cooperative launch
+// and come first in their work groups' job lists.
+__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
+                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
+                                                        unsigned long long *launch_ticks, const uint32_t *jobs) {
+  if (ranges) {
+    t0 = c4(ranges)[2u * blockIdx.x];
+    t1 = c4(ranges)[2u * blockIdx.x + 1u];
+  }
+  const unsigned long long launch_start = wall_clock64();
+  const uint32_t j0 = jobs ? c4(jobs)[blockIdx.x] : blockIdx.x;
+  const uint32_t j1 = jobs ? c4(jobs)[blockIdx.x + 1u] : blockIdx.x + 1u;
+  for (uint32_t j = j0; j < j1; ++j) {
+    run_group(groups + j, t0, t1, icm_lds, lds_bytes, launch_start, launch_ticks);
+    __syncthreads();
+  }
+}
+
 }  // namespace tamc
 
 // ------------------------------------------------------------------------ launch wrappers ----
+// Work groups of pt_sweep_kernel that can be resident at once on the device (a cooperative
+// launch must not exceed it).
+extern "C" hipError_t tamc_pt_sweep_capacity(uint32_t waves, uint32_t lds_bytes, uint32_t *capacity) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess && lds_bytes > 65536u)
+    e = hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)tamc::pt_sweep_kernel, (int)(waves * 64u),
+                                                     (size_t)lds_bytes);
+  *capacity = e == hipSuccess ? (uint32_t)(cus * per_cu) : 0u;
+  return e;
+}
+
 extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t ngroups,
                                           const uint32_t *const *init_bits, hipStream_t s) {
   hipLaunchKernelGGL(tamc::pt_init_kernel, dim3(ngroups), dim3(256), 0, s, groups, init_bits);
@@ -1624,7 +1756,8 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
 
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
-                                            unsigned long long *launch_ticks, hipStream_t s) {
+                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs,
+                                            int cooperative) {
   // dynamic LDS above 64 KiB must be requested explicitly; the attribute is raised to the largest
   // size any launch has used (callers may launch from several threads)
   static std::mutex mu;
@@ -1638,7 +1771,15 @@ extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_
       attr_bytes = lds_bytes;
     }
   }
+  if (cooperative) {
+    // every work group must be resident at once (team members wait for each other): the runtime
+    // refuses the launch otherwise
+    void *args[] = {(void *)&groups, (void *)&t0, (void *)&t1, (void *)&icm_lds, (void *)&lds_bytes,
+                    (void *)&ranges, (void *)&launch_ticks, (void *)&jobs};
+    return hipLaunchCooperativeKernel((const void *)tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), args,
+                                      lds_bytes, s);
+  }
   hipLaunchKernelGGL(tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1, icm_lds,
-                     lds_bytes, ranges, launch_ticks);
+                     lds_bytes, ranges, launch_ticks, jobs);
   return hipGetLastError();
 }

@@ -213,6 +213,10 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.gs_words = S.gs_words;
     X.pair_global = g.chain0 / 2;
     X.seed = S.P.seed;
+    X.members = 1;
+    X.member = 0;
+    X.team_ctr = nullptr;
+    X.team_box = nullptr;
   }
   return TAMC_OK;
 }
@@ -240,13 +244,129 @@ struct Launch {
   DevBuf d_groups, d_init_ptrs;
   std::vector<DevBuf *> init_bufs;
   uint32_t lds = 0, icm_lds = 1, max_T = 0;
+  // with teams: the sweep launch's descriptors (one per team member or group), grouped by work
+  // group through `jobs` (work group b runs desc[jobs[b]] .. desc[jobs[b + 1] - 1])
+  std::vector<PtGroup> desc;
+  std::vector<uint32_t> jobs;
+  DevBuf d_desc, d_jobs, d_team;
+  std::vector<size_t> team_ctr;      // word offset of each team's counters in d_team
+  uint32_t teams = 0;
   ~Launch() {
     for (auto *b : init_bufs) delete b;
   }
 };
 
+// Teams.  Per-sweep cost of a group in microseconds, from the measured phase split of the
+// batched kernel at 16 waves (random 3-regular graph, N = 3976, one plane word: Metropolis
+// 27.7 ns per site and word, ICM 12.3 ns per site and chunk of 32 pairs, 8 us of exchange,
+// measurement and barriers), scaled for the row width; a team of K pays a meeting of about 4 us
+// (arrival, wait, import of the other members' spin words) per colour class and one per sweep.
+static double sweep_cost_us(const SolvePlan &S, const GroupPlan &g, uint32_t K) {
+  const double n = (double)S.n, D = (double)std::max<uint32_t>(3u, (uint32_t)((S.inst->row_ptr[S.n] + S.n - 1) / S.n));
+  const double metro = 0.0277 * n * g.G * (1.0 + 0.15 * (D - 3.0));
+  const double icm = g.P ? 0.0123 * n * ((g.P + kMaxPairs - 1) / kMaxPairs) : 0.0;
+  const double levels = (double)S.inst->sched[S.P.sweep_order].n_classes;
+  return metro / K + icm + 8.0 + (K > 1 ? 4.0 * (levels + 1.0) : 0.0);
+}
+
+static bool team_eligible(const SolvePlan &S, const GroupPlan &g) {
+  return !g.icm_global && !(g.g.flags & kFlagIcmOnly) && g.g.self_J == nullptr && S.n >= 64u;
+}
+
+struct TeamJob {
+  uint32_t item, member, members;
+};
+
+// Assign groups to work groups: each group runs alone, or as a team of K members in K work groups,
+// and groups that run alone may follow one another in a work group.  For a target makespan M,
+// every group takes the smallest K whose cost fits M; team members open work groups, and the
+// single groups go, longest first, to the least loaded work group.  The smallest feasible M is
+// found by bisection.  force_k > 1 makes every eligible group a team of force_k.  Returns the
+// predicted makespan (us), or 0 when no plan fits the capacity.
+static double plan_teams(const std::vector<double> &cost1, const std::vector<std::vector<double>> &costk,
+                         const std::vector<uint32_t> &kvals, const std::vector<bool> &eligible, uint32_t capacity,
+                         uint32_t force_k, std::vector<std::vector<TeamJob>> &out) {
+  const size_t ni = cost1.size();
+  auto build = [&](double M, std::vector<std::vector<TeamJob>> *wg) -> double {
+    std::vector<uint32_t> K(ni, 1u);
+    uint32_t used = 0;
+    for (size_t i = 0; i < ni; ++i) {
+      if (!eligible[i]) continue;
+      if (force_k > 1u) {
+        K[i] = force_k;
+      } else if (cost1[i] > M) {
+        for (size_t q = 0; q < kvals.size(); ++q)
+          if (costk[i][q] <= M) {
+            K[i] = kvals[q];
+            break;
+          }
+      }
+      if (K[i] > 1u) used += K[i];
+    }
+    for (size_t i = 0; i < ni; ++i)
+      if (K[i] == 1u && cost1[i] > M && force_k <= 1u) return 0.0;
+    if (used > capacity) return 0.0;
+    std::vector<double> load;
+    std::vector<std::vector<TeamJob>> bins;
+    for (size_t i = 0; i < ni; ++i)
+      if (K[i] > 1u) {
+        const size_t q = std::find(kvals.begin(), kvals.end(), K[i]) - kvals.begin();
+        const double c = q < kvals.size() ? costk[i][q] : cost1[i];
+        for (uint32_t m = 0; m < K[i]; ++m) {
+          load.push_back(c);
+          bins.push_back({TeamJob{(uint32_t)i, m, K[i]}});
+        }
+      }
+    while (bins.size() < capacity) {
+      load.push_back(0.0);
+      bins.emplace_back();
+    }
+    std::vector<size_t> singles;
+    for (size_t i = 0; i < ni; ++i)
+      if (K[i] == 1u) singles.push_back(i);
+    std::sort(singles.begin(), singles.end(), [&](size_t a, size_t b) { return cost1[a] > cost1[b]; });
+    for (size_t i : singles) {
+      const size_t b = std::min_element(load.begin(), load.end()) - load.begin();
+      load[b] += cost1[i];
+      bins[b].push_back(TeamJob{(uint32_t)i, 0u, 1u});
+    }
+    double mk = 0.0;
+    for (double l : load) mk = std::max(mk, l);
+    if (wg) {
+      wg->clear();
+      for (auto &b : bins)
+        if (!b.empty()) wg->push_back(b);
+    }
+    return mk;
+  };
+  if (force_k > 1u) return build(0.0, &out);
+  double hi = 0.0, lo = 0.0;
+  for (size_t i = 0; i < ni; ++i) {
+    hi = std::max(hi, cost1[i]);
+    double best = cost1[i];
+    if (eligible[i])
+      for (double c : costk[i]) best = std::min(best, c);
+    lo = std::max(lo, best);
+  }
+  double best_mk = 0.0;
+  for (int it = 0; it < 40 && hi - lo > 0.01; ++it) {
+    const double M = 0.5 * (lo + hi);
+    const double mk = build(M, nullptr);
+    if (mk > 0.0 && mk <= M * 1.0001) {
+      hi = M;
+    } else {
+      lo = M;
+    }
+  }
+  best_mk = build(hi, &out);
+  return best_mk;
+}
+
+
+// teams: 0 = none, 1 = where the cost model predicts a makespan at least 10 % shorter, K >= 2 =
+// every eligible group as a team of K work groups
 static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves, uint32_t tiling, uint32_t growth,
-                                  Launch &Lc) {
+                                  uint32_t teams, Launch &Lc) {
   struct Item {
     SolvePlan *S;
     GroupPlan *g;
@@ -286,6 +406,75 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     TAMC_HIP(hipMemcpy(Lc.d_init_ptrs.p, Lc.init_ptrs.data(), Lc.init_ptrs.size() * sizeof(void *),
                        hipMemcpyHostToDevice));
   }
+  if (!teams) return TAMC_OK;
+  uint32_t capacity = 0;
+  if (tamc_pt_sweep_capacity(waves, Lc.lds, &capacity) != hipSuccess || capacity < 2u || items.size() > capacity) {
+    (void)hipGetLastError();
+    return TAMC_OK;  // no room for teams: every group runs alone
+  }
+  std::vector<uint32_t> kvals;
+  for (uint32_t k : {2u, 3u, 4u, 5u, 6u, 8u, 10u, 12u, 16u})
+    if (k <= kMaxMembers && k <= capacity) kvals.push_back(k);
+  const size_t ni = items.size();
+  std::vector<double> cost1(ni);
+  std::vector<std::vector<double>> costk(ni);
+  std::vector<bool> eligible(ni);
+  bool any = false;
+  for (size_t i = 0; i < ni; ++i) {
+    const SolvePlan &S = *items[i].S;
+    const GroupPlan &g = *items[i].g;
+    cost1[i] = sweep_cost_us(S, g, 1u) * S.T;
+    for (uint32_t k : kvals) costk[i].push_back(sweep_cost_us(S, g, k) * S.T);
+    eligible[i] = team_eligible(S, g);
+    any |= eligible[i];
+  }
+  if (!any) return TAMC_OK;
+  std::vector<std::vector<TeamJob>> wgs;
+  const uint32_t force = teams >= 2u ? std::min(teams, std::min(kMaxMembers, capacity)) : 0u;
+  const double mk = plan_teams(cost1, costk, kvals, eligible, capacity, force, wgs);
+  double base = 0.0;
+  for (double c : cost1) base = std::max(base, c);
+  bool used = false;
+  for (auto &w : wgs)
+    for (auto &j : w) used |= j.members > 1u;
+  if (!used || mk <= 0.0 || (!force && mk > 0.9 * base) || wgs.size() > capacity) return TAMC_OK;
+  // team scratch: counters (64-byte aligned) and the spin-word / energy-sum box of each team
+  std::vector<size_t> ctr_off(ni, 0), box_off(ni, 0);
+  std::vector<uint32_t> members(ni, 1u);
+  for (auto &w : wgs)
+    for (auto &j : w) members[j.item] = j.members;
+  size_t words = 0;  // in 8-byte units
+  for (size_t i = 0; i < ni; ++i) {
+    if (members[i] < 2u) continue;
+    ctr_off[i] = words;
+    words += (kTeamCtrWords * 4u + 63u) / 64u * 8u;
+    box_off[i] = words;
+    words += (size_t)items[i].S->n * items[i].g->G + (size_t)members[i] * items[i].g->R;
+    words = (words + 7u) & ~(size_t)7u;
+    ++Lc.teams;
+  }
+  TAMC_HIP(Lc.d_team.alloc(words * 8u));
+  TAMC_HIP(hipMemset(Lc.d_team.p, 0, words * 8u));
+  unsigned long long *tb = (unsigned long long *)Lc.d_team.p;
+  Lc.jobs.push_back(0u);
+  for (auto &w : wgs) {
+    for (auto &j : w) {
+      PtGroup d = Lc.order[j.item];
+      if (j.members > 1u) {
+        d.members = j.members;
+        d.member = j.member;
+        d.team_ctr = (uint32_t *)(tb + ctr_off[j.item]);
+        d.team_box = tb + box_off[j.item];
+        if (j.member == 0u) Lc.team_ctr.push_back(ctr_off[j.item] * 2u);
+      }
+      Lc.desc.push_back(d);
+    }
+    Lc.jobs.push_back((uint32_t)Lc.desc.size());
+  }
+  TAMC_HIP(Lc.d_desc.alloc(Lc.desc.size() * sizeof(PtGroup)));
+  TAMC_HIP(hipMemcpy(Lc.d_desc.p, Lc.desc.data(), Lc.desc.size() * sizeof(PtGroup), hipMemcpyHostToDevice));
+  TAMC_HIP(Lc.d_jobs.alloc(Lc.jobs.size() * 4u));
+  TAMC_HIP(hipMemcpy(Lc.d_jobs.p, Lc.jobs.data(), Lc.jobs.size() * 4u, hipMemcpyHostToDevice));
   return TAMC_OK;
 }
 
@@ -470,6 +659,7 @@ struct RunOptions {
   uint32_t tiling = 1;             // 1 tasks dealt to the waves in turn, 2 contiguous runs
   uint32_t growth = 0;             // ICM cluster growth: 1 strided sites per thread, 2 compact tiles,
                                    // 0 each group's default (tiles for rows wider than 4 entries)
+  uint32_t teams = 1;              // 0 none, 1 where the cost model gains, K >= 2 forced teams of K
   bool use_graph = false;
   std::chrono::steady_clock::time_point t_entry = std::chrono::steady_clock::now();  // for prep_ms
 };
@@ -482,7 +672,7 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (st) return st;
   if (patch) patch(solves);
   Launch Lc;
-  if ((st = prepare_launch(solves, opt.waves, opt.tiling, opt.growth, Lc))) return st;
+  if ((st = prepare_launch(solves, opt.waves, opt.tiling, opt.growth, opt.teams, Lc))) return st;
   const uint32_t ng = (uint32_t)Lc.order.size();
   hipStream_t s;
   TAMC_HIP(hipStreamCreate(&s));
@@ -502,18 +692,26 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   uint32_t launches = 0;
   const auto w0 = std::chrono::steady_clock::now();
   hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  hipGraph_t hgraph = nullptr;
   TAMC_HIP(hipEventRecord(e0, s));
-  if (opt.use_graph) {
+  const bool graph = opt.use_graph && !Lc.teams;  // team launches are cooperative: launched directly
+  if (Lc.teams) {
+    const PtGroup *dd = (const PtGroup *)Lc.d_desc.p;
+    const uint32_t nwg = (uint32_t)Lc.jobs.size() - 1u;
+    for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
+      TAMC_HIP(tamc_launch_pt_sweeps(dd, nwg, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s,
+                                     (const uint32_t *)Lc.d_jobs.p, 1));
+  } else if (graph) {
     TAMC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s);
-    TAMC_HIP(hipStreamEndCapture(s, &graph));
-    TAMC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, nullptr, 0);
+    TAMC_HIP(hipStreamEndCapture(s, &hgraph));
+    TAMC_HIP(hipGraphInstantiate(&gexec, hgraph, nullptr, nullptr, 0));
     TAMC_HIP(hipGraphLaunch(gexec, s));
   } else {
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s));
+      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, nullptr,
+                                     0));
   }
   TAMC_HIP(hipEventRecord(e1, s));
   TAMC_HIP(hipEventSynchronize(e1));
@@ -523,7 +721,12 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
+  if (hgraph) (void)hipGraphDestroy(hgraph);
+  for (size_t off : Lc.team_ctr) {
+    uint32_t ab = 0;
+    TAMC_HIP(hipMemcpy(&ab, (const uint32_t *)Lc.d_team.p + off + 1u, 4, hipMemcpyDeviceToHost));
+    if (ab) return fail(TAMC_E_DEVICE, "a team of work groups gave up waiting for its members");
+  }
   const double wall_us = std::chrono::duration<double, std::micro>(w1 - w0).count();
   uint64_t attempts = 0;
   std::vector<char> mirror_buf(A.used);
@@ -589,7 +792,9 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
     stats->groups = ng;
     stats->task_tiling = opt.tiling;
     stats->icm_growth = opt.growth;
-    stats->graph = opt.use_graph ? 1 : 0;
+    stats->graph = graph ? 1 : 0;
+    stats->teams = Lc.teams;
+    stats->work_groups = Lc.teams ? (uint32_t)Lc.jobs.size() - 1u : ng;
     stats->prep_ms = std::chrono::duration<double, std::milli>(w0 - opt.t_entry).count();
     stats->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w1).count();
   }
@@ -690,14 +895,15 @@ tamc_status tamc_pt_run_resume(const tamc_instance *inst, const tamc_pt_params *
 // per batch shape for the life of the process.
 namespace {
 struct TuneKey {
-  uint32_t count, max_n, B, C, order;
+  uint32_t count, max_n, B, C, order, fixed_teams;
   uint64_t sum_n;
   bool operator<(const TuneKey &o) const {
-    return std::tie(count, max_n, B, C, order, sum_n) < std::tie(o.count, o.max_n, o.B, o.C, o.order, o.sum_n);
+    return std::tie(count, max_n, B, C, order, fixed_teams, sum_n) <
+           std::tie(o.count, o.max_n, o.B, o.C, o.order, o.fixed_teams, o.sum_n);
   }
 };
 struct TuneChoice {
-  uint32_t waves = 16, tiling = 1, growth = 0;
+  uint32_t waves = 16, tiling = 1, growth = 0, teams = 0;
   bool quarters = false;  // a quarter of the sweeps per launch, launches captured in a hipGraph
 };
 std::mutex g_tune_mu;
@@ -705,10 +911,13 @@ std::map<TuneKey, TuneChoice> g_tune_cache;
 constexpr uint32_t kTunePrefix = 24u;
 }  // namespace
 
+// fixed_teams: the RunOptions value the caller fixed (0 none, K >= 2 forced), or kTeamsTuned
+constexpr uint32_t kTeamsTuned = 0xFFFFFFFFu;
+
 static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *insts, const tamc_pt_params *params,
                                   uint32_t fixed_waves, uint32_t fixed_tiling, uint32_t fixed_growth,
-                                  TuneChoice *out) {
-  TuneKey key{count, 0, 0, 0, 0, 0};
+                                  uint32_t fixed_teams, TuneChoice *out) {
+  TuneKey key{count, 0, 0, 0, 0, fixed_teams, 0};
   for (uint32_t k = 0; k < count; ++k) {
     key.order |= 1u << params[k].sweep_order;
     key.max_n = std::max(key.max_n, insts[k]->n);
@@ -750,6 +959,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
     return TAMC_OK;
   };
   TuneChoice best;
+  const uint32_t base_teams = fixed_teams == kTeamsTuned ? 0u : fixed_teams;
   double best_ms = 1e300;
   bool any = false;
   for (uint32_t w : {4u, 8u, 16u}) {
@@ -760,6 +970,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
       opt.waves = w;
       opt.tiling = tl;
       opt.growth = fixed_growth;
+      opt.teams = base_teams;
       double ms;
       const tamc_status st = time_one(opt, &ms);
       if (st) return st;
@@ -782,6 +993,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
     opt.waves = best.waves;
     opt.tiling = best.tiling;
     opt.growth = gr;
+    opt.teams = base_teams;
     double ms;
     const tamc_status st = time_one(opt, &ms);
     if (st) return st;
@@ -790,11 +1002,31 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
       best.growth = gr;
     }
   }
+  best.teams = base_teams;
+  if (any && fixed_teams == kTeamsTuned) {
+    // teams where the cost model places them, kept only when clearly faster (timed twice: the
+    // first cooperative launch of a process pays a one-time set-up)
+    RunOptions opt;
+    opt.waves = best.waves;
+    opt.tiling = best.tiling;
+    opt.growth = best.growth;
+    opt.teams = 1u;
+    double ms, ms2;
+    tamc_status st = time_one(opt, &ms);
+    if (st) return st;
+    if ((st = time_one(opt, &ms2))) return st;
+    ms = std::min(ms, ms2);
+    if (ms >= 0.0 && ms < 0.97 * best_ms) {
+      best_ms = ms;
+      best.teams = 1u;
+    }
+  }
   if (any) {
     RunOptions opt;
     opt.waves = best.waves;
     opt.tiling = best.tiling;
     opt.growth = best.growth;
+    opt.teams = best.teams;
     opt.sweeps_per_launch = (kTunePrefix + 3u) / 4u;
     opt.use_graph = true;
     double ms;
@@ -831,19 +1063,26 @@ tamc_status tamc_pt_run_batch_init(uint32_t count, const tamc_instance *const *i
   if (w < 1 || w > kMaxWaves) return fail(TAMC_E_INVALID_ARG, "waves_per_group must be 1..16");
   if (opts && opts->task_tiling > 2u) return fail(TAMC_E_INVALID_ARG, "task_tiling must be 0, 1 or 2");
   if (opts && opts->icm_growth > 2u) return fail(TAMC_E_INVALID_ARG, "icm_growth must be 0, 1 or 2");
+  if (opts && opts->teams > kMaxMembers) return fail(TAMC_E_INVALID_ARG, "teams must be 0..16");
   opt.waves = w;
   opt.tiling = (opts && opts->task_tiling) ? opts->task_tiling : 1u;
   opt.growth = opts ? opts->icm_growth : 0u;
+  // options value -> run value: 0 the cost model, 1 none, K forced teams of K
+  const uint32_t teams_opt = opts ? opts->teams : 0u;
+  opt.teams = teams_opt == 0u ? 1u : (teams_opt == 1u ? 0u : teams_opt);
   opt.sweeps_per_launch = opts ? opts->sweeps_per_launch : 0;
   opt.use_graph = opts && opts->use_graph;
   if (opts && opts->autotune && count &&
-      (!opts->waves_per_group || !opts->task_tiling || !opts->icm_growth || !opts->sweeps_per_launch)) {
+      (!opts->waves_per_group || !opts->task_tiling || !opts->icm_growth || !opts->teams ||
+       !opts->sweeps_per_launch)) {
     TuneChoice tc;
-    if ((st = autotune_batch(count, insts, params, opts->waves_per_group, opts->task_tiling, opts->icm_growth, &tc)))
+    if ((st = autotune_batch(count, insts, params, opts->waves_per_group, opts->task_tiling, opts->icm_growth,
+                             teams_opt == 0u ? kTeamsTuned : opt.teams, &tc)))
       return st;
     opt.waves = tc.waves;
     opt.tiling = tc.tiling;
     opt.growth = tc.growth;
+    opt.teams = tc.teams;
     if (!opts->sweeps_per_launch && tc.quarters) {
       uint32_t maxT = 0;
       for (uint32_t k = 0; k < count; ++k) maxT = std::max(maxT, params[k].num_sweeps);
@@ -882,6 +1121,8 @@ tamc_status tamc_pt_run_plain(uint32_t count, const tamc_instance *const *insts,
     acc.launches += one.launches;
     acc.groups += one.groups;
     acc.groups_busy_ms += one.groups_busy_ms;
+    acc.teams += one.teams;
+    acc.work_groups = std::max(acc.work_groups, one.work_groups);
   }
   if (stats) *stats = acc;
   return TAMC_OK;

@@ -97,7 +97,19 @@ struct PtGroup {
   // ---- per-phase time of the group (100 MHz wall clock ticks): ICM overlap+seeds, ICM cluster growth,
   //      ICM flip, exchange, sweep, measurement ----
   unsigned long long *phase_ticks;  // [kPhaseSlots]
+  // ---- teams: a group whose Metropolis sweep is shared by `members` work groups of one
+  //      cooperative launch.  Every member holds the whole state in its LDS and runs ICM and the
+  //      exchange itself (the same inputs give the same results); member m updates the RNG
+  //      groups g with g % members == m, publishes the new spin words and its share of the
+  //      accepted energy through team_box, and meets the others at a counter after each colour
+  //      class and each sweep.  Member 0 alone writes measurements and the final state. ----
+  uint32_t members, member;  // 1, 0 for a group run by one work group
+  uint32_t *team_ctr;        // [kTeamCtrWords]: arrivals, abort flag, -, epochs of member 0, 1, ...
+  unsigned long long *team_box;  // [n * G] spin words, then [members][R] accepted-energy sums
 };
+
+constexpr uint32_t kMaxMembers = 16;
+constexpr uint32_t kTeamCtrWords = 3 + kMaxMembers;
 
 struct LdsLayout {
   uint32_t spins, energy, beta_of, perm, tags, nacc, hist, rt, pair_a, pair_b, pair_of, seed_blk, misc, ticks, fmask, fscr, partial, dE, drift,
@@ -140,7 +152,8 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(R * 4u);
   L.seed_blk = o;
   o += lds_align(kMaxPairs * 4u);
-  L.misc = o;
+  L.misc = o;  // [0..1] measurement, [4] ICM moves, [8..10] growth pass flags, [12..14] round flags
+                // of block_any, [15] team meetings
   o += lds_align(16u * 4u);
   L.ticks = o;  // phase timers [kPhaseSlots] + last timestamp
   o += lds_align((kPhaseSlots + 1u) * 8u);

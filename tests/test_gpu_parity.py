@@ -296,9 +296,42 @@ def test_batch_equals_single_and_plain(gpu):
     assert rst["task_tiling"] == 2 and rst["launches"] == 4 and rst["graph"]
     strided, sst = api.run_pt_batch(insts, params, icm_growth=1)
     assert sst["icm_growth"] == 1
+    teamed, tmst = api.run_pt_batch(insts, params, teams=3)
+    assert tmst["teams"] == 3  # the three instances with at least 64 sites
     for k in range(len(insts)):
         single = api.PtIcmRunner(insts[k], params[k]).run()
-        for other in (plain[k], chunked[k], tuned[k], runs[k], strided[k], single):
+        for other in (plain[k], chunked[k], tuned[k], runs[k], strided[k], teamed[k], single):
             for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
                 assert np.array_equal(batch[k][key], other[key]), key
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
+
+
+@pytest.mark.parametrize("K", [2, 4, 16])
+def test_teams_share_the_sweep(gpu, K):
+    # a group's Metropolis sweep shared by K work groups that exchange spin words after every colour
+    # class: the same results as one work group, also over several launches, against the oracle
+    cases = [("3d_pm", dict(L=10)), ("rr3", dict(n=3000)), ("2d_pm", dict(L=24)), ("rr3_pm", dict(n=500))]
+    insts, oracles, params = [], [], []
+    for k, (kind, kw) in enumerate(cases):
+        oi, gi, _ = make_pair(kind, seed=70 + k, **kw)
+        insts.append(gi)
+        oracles.append(oi)
+        params.append(api.PtIcmParams(num_sweeps=12, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 12),
+                                      lo_num_beta=6, icm=True, num_replica_chains=2 if k != 2 else 4,
+                                      threads=1 if k != 3 else 2, sample=2, sample_states=3, sample_limiting=2,
+                                      seed=700 + k))
+    alone, ast = api.run_pt_batch(insts, params, teams=1)
+    assert ast["teams"] == 0
+    team, tst = api.run_pt_batch(insts, params, teams=K)
+    assert tst["teams"] == 5 and tst["work_groups"] >= 2 * K
+    chunked, _ = api.run_pt_batch(insts, params, teams=K, sweeps_per_launch=5)
+    for k, (oi, p) in enumerate(zip(oracles, params)):
+        runner = api.PtIcmRunner(insts[k], p)
+        o = O.pt_run(oi, runner.betas, 12, num_chains=p.num_replica_chains, warmup_fraction=0.5,
+                     lo_beta_idx=runner.lo_beta_idx, icm=True, threaded=p.threads > 1, sample=2, sample_states=3,
+                     sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
+        assert o["error"] == 0
+        assert_same(team[k], o, exact_energy=True, ctx=(K, k))
+        for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
+            assert np.array_equal(alone[k][key], team[k][key]), (K, k, key)
+            assert np.array_equal(chunked[k][key], team[k][key]), (K, k, key)
