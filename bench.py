@@ -309,6 +309,7 @@ def main():
     ap.add_argument("--sweeps-per-launch", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true", help="launch without hipGraph capture")
     ap.add_argument("--tiling", type=int, default=0, help="task tiling (1 dealt in turn, 2 runs; 0 = autotuned)")
+    ap.add_argument("--teams", type=int, default=0, help="teams of work groups (1 never, K forced; 0 = autotuned)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the other configurations")
@@ -359,7 +360,8 @@ def main():
 
     def step():
         return api.run_pt_batch(insts, params, waves_per_group=args.waves, sweeps_per_launch=args.sweeps_per_launch,
-                                use_graph=not args.no_graph, autotune=True, task_tiling=args.tiling)
+                                use_graph=not args.no_graph, autotune=True, task_tiling=args.tiling,
+                                teams=args.teams)
 
     for _ in range(args.warmup):
         step()

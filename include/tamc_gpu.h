@@ -237,11 +237,11 @@ typedef struct {
                                 whose rows hold more than 4 couplings and strides otherwise.  The
                                 clusters do not depend on it */
   uint32_t teams;            /* a large group's Metropolis sweep shared by a team of work groups of
-                                one cooperative launch (small groups then share work groups, one
-                                after another): 0 = where a cost model predicts a shorter launch
-                                (with `autotune`: kept only if it times faster), 1 = never, K >= 2
-                                = every eligible group as a team of K (at most 16).  Results do not
-                                depend on it */
+                                one launch, all resident at once (small groups then share work
+                                groups, one after another): 0 = where a cost model predicts a
+                                shorter launch (with `autotune`: kept only if it times faster),
+                                1 = never, K >= 2 = every eligible group as a team of K (at most
+                                16).  Results do not depend on it */
 } tamc_batch_options;
 
 typedef struct {

@@ -15,8 +15,7 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
                                           const uint32_t *const *init_bits, hipStream_t s);
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
-                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs,
-                                            int cooperative);
+                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs);
 extern "C" hipError_t tamc_pt_sweep_capacity(uint32_t waves, uint32_t lds_bytes, uint32_t *capacity);
 
 namespace tamc {

@@ -1369,6 +1369,18 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
       atomicAdd(acc + g * 64u, (unsigned long long)to_fix(a_g));
     }
     if (K > 1u) {
+      // after the last level, this member's share of the accepted energy goes out with its spin
+      // words, so that one meeting serves both
+      unsigned long long *sums = X.team_box + (size_t)c.n * G;
+      if (l + 1u == nl) {
+        __syncthreads();
+        for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
+          const uint32_t g = r >> 6, ln = r & 63u;
+          long long s = 0;
+          for (uint32_t w = 0; w < c.W; ++w) s += c.partial[(w * G + g) * 64u + ln];
+          box_store(sums + (size_t)c.member * c.R + r, (unsigned long long)s);
+        }
+      }
       // take the other members' new spin words of this level
       team_sync(c);
       const uint32_t items = (gb - ga) * 4u * G;
@@ -1386,14 +1398,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   __syncthreads();
   if (K > 1u) {
     // every member adds every member's share, in member order: the same energies everywhere
-    unsigned long long *sums = X.team_box + (size_t)c.n * G;
-    for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
-      const uint32_t g = r >> 6, l = r & 63u;
-      long long s = 0;
-      for (uint32_t w = 0; w < c.W; ++w) s += c.partial[(w * G + g) * 64u + l];
-      box_store(sums + (size_t)c.member * c.R + r, (unsigned long long)s);
-    }
-    team_sync(c);
+    const unsigned long long *sums = X.team_box + (size_t)c.n * G;
     for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
       long long s = 0;
       for (uint32_t m = 0; m < K; ++m) s += (long long)box_load(sums + (size_t)m * c.R + r);
@@ -1711,8 +1716,8 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
 }
 
 // Work group b runs the groups jobs[b] .. jobs[b + 1] - 1 one after another (without a job table,
-// group b alone).  The members of a team are in different work groups of one cooperative launch
-// and come first in their work groups' job lists.
+// group b alone).  The members of a team are in different work groups of one launch, all resident
+// at once, and come first in their work groups' job lists.
 __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
                                                         uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
                                                         unsigned long long *launch_ticks, const uint32_t *jobs) {
@@ -1732,8 +1737,8 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
 }  // namespace tamc
 
 // ------------------------------------------------------------------------ launch wrappers ----
-// Work groups of pt_sweep_kernel that can be resident at once on the device (a cooperative
-// launch must not exceed it).
+// Work groups of pt_sweep_kernel that can be resident at once on the device (a launch with teams
+// must not exceed it).
 extern "C" hipError_t tamc_pt_sweep_capacity(uint32_t waves, uint32_t lds_bytes, uint32_t *capacity) {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -1756,8 +1761,7 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
 
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
-                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs,
-                                            int cooperative) {
+                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs) {
   // dynamic LDS above 64 KiB must be requested explicitly; the attribute is raised to the largest
   // size any launch has used (callers may launch from several threads)
   static std::mutex mu;
@@ -1771,14 +1775,11 @@ extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_
       attr_bytes = lds_bytes;
     }
   }
-  if (cooperative) {
-    // every work group must be resident at once (team members wait for each other): the runtime
-    // refuses the launch otherwise
-    void *args[] = {(void *)&groups, (void *)&t0, (void *)&t1, (void *)&icm_lds, (void *)&lds_bytes,
-                    (void *)&ranges, (void *)&launch_ticks, (void *)&jobs};
-    return hipLaunchCooperativeKernel((const void *)tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), args,
-                                      lds_bytes, s);
-  }
+  // With teams (jobs set), members wait for each other, so every work group must be resident at
+  // once: the planner keeps the grid within tamc_pt_sweep_capacity, which an otherwise idle device
+  // dispatches in full, and the members' waits are bounded, so a shortfall ends the launch with an
+  // error instead of a hang.  (A cooperative launch would guarantee residency, but the profiler
+  // does not finish processes that use one.)
   hipLaunchKernelGGL(tamc::pt_sweep_kernel, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1, icm_lds,
                      lds_bytes, ranges, launch_ticks, jobs);
   return hipGetLastError();

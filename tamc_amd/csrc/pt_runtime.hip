@@ -260,17 +260,21 @@ struct Launch {
 // batched kernel at 16 waves (random 3-regular graph, N = 3976, one plane word: Metropolis
 // 27.7 ns per site and word, ICM 12.3 ns per site and chunk of 32 pairs, 8 us of exchange,
 // measurement and barriers), scaled for the row width; a team of K pays a meeting of about 4 us
-// (arrival, wait, import of the other members' spin words) per colour class and one per sweep.
+// (arrival, wait, import of the other members' spin words) per colour class.
 static double sweep_cost_us(const SolvePlan &S, const GroupPlan &g, uint32_t K) {
   const double n = (double)S.n, D = (double)std::max<uint32_t>(3u, (uint32_t)((S.inst->row_ptr[S.n] + S.n - 1) / S.n));
   const double metro = 0.0277 * n * g.G * (1.0 + 0.15 * (D - 3.0));
   const double icm = g.P ? 0.0123 * n * ((g.P + kMaxPairs - 1) / kMaxPairs) : 0.0;
   const double levels = (double)S.inst->sched[S.P.sweep_order].n_classes;
-  return metro / K + icm + 8.0 + (K > 1 ? 4.0 * (levels + 1.0) : 0.0);
+  return metro / K + icm + 8.0 + (K > 1 ? 4.0 * levels : 0.0);
 }
 
+// Teams need a group whose ICM scratch is in LDS (members keep private copies), no QUBO self
+// entries (their drift is per flip) and at least two colour classes: a member rewrites a class's
+// exchange words only after the next meeting, which every member reaches after reading them.
 static bool team_eligible(const SolvePlan &S, const GroupPlan &g) {
-  return !g.icm_global && !(g.g.flags & kFlagIcmOnly) && g.g.self_J == nullptr && S.n >= 64u;
+  return !g.icm_global && !(g.g.flags & kFlagIcmOnly) && g.g.self_J == nullptr && S.n >= 64u &&
+         S.inst->sched[S.P.sweep_order].n_classes >= 2u;
 }
 
 struct TeamJob {
@@ -694,24 +698,21 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   hipGraphExec_t gexec = nullptr;
   hipGraph_t hgraph = nullptr;
   TAMC_HIP(hipEventRecord(e0, s));
-  const bool graph = opt.use_graph && !Lc.teams;  // team launches are cooperative: launched directly
-  if (Lc.teams) {
-    const PtGroup *dd = (const PtGroup *)Lc.d_desc.p;
-    const uint32_t nwg = (uint32_t)Lc.jobs.size() - 1u;
-    for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      TAMC_HIP(tamc_launch_pt_sweeps(dd, nwg, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s,
-                                     (const uint32_t *)Lc.d_jobs.p, 1));
-  } else if (graph) {
+  const bool graph = opt.use_graph;
+  // with teams, the work groups run the descriptors their job lists name
+  const PtGroup *lg = Lc.teams ? (const PtGroup *)Lc.d_desc.p : dg;
+  const uint32_t lng = Lc.teams ? (uint32_t)Lc.jobs.size() - 1u : ng;
+  const uint32_t *ljobs = Lc.teams ? (const uint32_t *)Lc.d_jobs.p : nullptr;
+  if (graph) {
     TAMC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      (void)tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, nullptr, 0);
+      (void)tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs);
     TAMC_HIP(hipStreamEndCapture(s, &hgraph));
     TAMC_HIP(hipGraphInstantiate(&gexec, hgraph, nullptr, nullptr, 0));
     TAMC_HIP(hipGraphLaunch(gexec, s));
   } else {
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      TAMC_HIP(tamc_launch_pt_sweeps(dg, ng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, nullptr,
-                                     0));
+      TAMC_HIP(tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs));
   }
   TAMC_HIP(hipEventRecord(e1, s));
   TAMC_HIP(hipEventSynchronize(e1));
@@ -1005,7 +1006,7 @@ static tamc_status autotune_batch(uint32_t count, const tamc_instance *const *in
   best.teams = base_teams;
   if (any && fixed_teams == kTeamsTuned) {
     // teams where the cost model places them, kept only when clearly faster (timed twice: the
-    // first cooperative launch of a process pays a one-time set-up)
+    // first team launch of a process pays a one-time set-up)
     RunOptions opt;
     opt.waves = best.waves;
     opt.tiling = best.tiling;

@@ -97,12 +97,13 @@ struct PtGroup {
   // ---- per-phase time of the group (100 MHz wall clock ticks): ICM overlap+seeds, ICM cluster growth,
   //      ICM flip, exchange, sweep, measurement ----
   unsigned long long *phase_ticks;  // [kPhaseSlots]
-  // ---- teams: a group whose Metropolis sweep is shared by `members` work groups of one
-  //      cooperative launch.  Every member holds the whole state in its LDS and runs ICM and the
-  //      exchange itself (the same inputs give the same results); member m updates the RNG
-  //      groups g with g % members == m, publishes the new spin words and its share of the
-  //      accepted energy through team_box, and meets the others at a counter after each colour
-  //      class and each sweep.  Member 0 alone writes measurements and the final state. ----
+  // ---- teams: a group whose Metropolis sweep is shared by `members` work groups of one launch,
+  //      all resident at once.  Every member holds the whole state in its LDS and runs ICM and
+  //      the exchange itself (the same inputs give the same results); member m updates the RNG
+  //      groups g with g % members == m, publishes the new spin words (and, after the last colour
+  //      class, its share of the accepted energy) through team_box, and meets the others at a
+  //      counter after each colour class.  Member 0 alone writes measurements and the final
+  //      state. ----
   uint32_t members, member;  // 1, 0 for a group run by one work group
   uint32_t *team_ctr;        // [kTeamCtrWords]: arrivals, abort flag, -, epochs of member 0, 1, ...
   unsigned long long *team_box;  // [n * G] spin words, then [members][R] accepted-energy sums

@@ -532,8 +532,7 @@ struct tamc_server {
       for (auto &bo : offsets[s]) {
         Bucket *b = bo.first;
         TAMC_HIP(tamc_launch_pt_sweeps(d_groups[s] + bo.second, (uint32_t)b->members.size(), 0, 0, b->waves, b->lds, 1u,
-                                       d_ranges[s] + 2 * bo.second, d_ticks[s] + bo.second, streams[s], nullptr,
-                                       0));
+                                       d_ranges[s] + 2 * bo.second, d_ticks[s] + bo.second, streams[s], nullptr));
       }
       if (sa_here) {
         TAMC_HIP(hipEventRecord(ev_sa0, streams[s]));
