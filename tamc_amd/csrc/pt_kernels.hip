@@ -425,9 +425,13 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       }
     }
   };
-  if (c.nblk >= c.W) {
+  // the work group's blocks: all, or in a team the member's blocks member, member + K, ...
+  const uint32_t K = c.members;
+  const uint32_t nbm = c.nblk > c.member ? (c.nblk - c.member + K - 1u) / K : 0u;
+  if (nbm >= c.W) {
     // a wave owns whole blocks: terms, then flips, block after block
-    for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+    for (uint32_t j = c.wave; j < nbm; j += c.W) {
+      const uint32_t blk = c.member + j * K;
       const uint32_t i_lane = blk * 64u + c.lane;
       const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
       if (!ballot(cwi != 0u)) continue;
@@ -439,18 +443,54 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
     // flips wait until every term is done
     uint32_t pmask = 0u;
     for (uint32_t p = c.wave; p < P; p += c.W) pmask |= 1u << p;
-    for (uint32_t blk = 0; blk < c.nblk; ++blk) {
+    for (uint32_t j = 0; j < nbm; ++j) {
+      const uint32_t blk = c.member + j * K;
       const uint32_t i_lane = blk * 64u + c.lane;
       const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
       terms(blk, cwi, pmask);
     }
     __syncthreads();
-    for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+    for (uint32_t j = c.wave; j < nbm; j += c.W) {
+      const uint32_t blk = c.member + j * K;
       const uint32_t i_lane = blk * 64u + c.lane;
       const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
       if (ballot(cwi != 0u)) flip_block(blk, cwi);
     }
   }
+}
+
+// In a team every member flips its own 64-site blocks and sums their terms; the members then
+// publish the spin words of their blocks and their per-pair sums, meet, and take the others'.
+// Two buffers alternate by ICM chunk, so a member that runs ahead into the next chunk does not
+// overwrite words another member is still reading.
+__device__ __forceinline__ void icm_team_merge(Ctx &c) {
+  const TAMC_AS4 PtGroup &X = *c.X;
+  const uint32_t K = c.members, G = c.G, n = c.n, P = c.P;
+  const size_t words = (size_t)n * G;
+  const uint32_t par = ((c.lo - X.lo) / kMaxPairs) & 1u;
+  unsigned long long *F = X.team_box + words + (size_t)K * c.R + par * (words + (size_t)K * kMaxPairs);
+  unsigned long long *Dp = F + words;
+  __syncthreads();
+  const uint32_t nbm = c.nblk > c.member ? (c.nblk - c.member + K - 1u) / K : 0u;
+  for (uint32_t idx = c.tid; idx < nbm * 64u * G; idx += c.nthreads) {
+    const uint32_t j = idx / (64u * G), rem = idx - j * 64u * G;
+    const uint32_t i = (c.member + j * K) * 64u + rem / G, g = rem % G;
+    if (i < n) box_store(F + (size_t)i * G + g, *reinterpret_cast<const uint64_t *>(c.sp + (i * G + g) * 2u));
+  }
+  for (uint32_t p = c.tid; p < P; p += c.nthreads)
+    box_store(Dp + (size_t)c.member * kMaxPairs + p, (unsigned long long)c.dEa[p]);
+  team_sync(c);
+  for (uint32_t idx = c.tid; idx < n * G; idx += c.nthreads) {
+    const uint32_t i = idx / G;
+    if ((i / 64u) % K == c.member) continue;
+    *reinterpret_cast<uint64_t *>(c.sp + idx * 2u) = box_load(F + idx);
+  }
+  for (uint32_t p = c.tid; p < P; p += c.nthreads) {
+    long long s = 0;
+    for (uint32_t m = 0; m < K; ++m) s += (long long)box_load(Dp + (size_t)m * kMaxPairs + p);
+    c.dEa[p] = s;
+  }
+  __syncthreads();
 }
 
 // ICM step (c) for the pairs whose clusters still grow after kLabelAfterPasses passes (long
@@ -1041,6 +1081,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     case 8u: icm_flip<8>(c, M); break;
     default: icm_flip<0>(c, M);
   }
+  if (c.members > 1u) icm_team_merge(c);
   __syncthreads();
   // moved pairs: energies become energy_ref of the new states, as the reference recomputes them
   if (c.tid < P && c.seed_blk[c.tid] != kNone) {

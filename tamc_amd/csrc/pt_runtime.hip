@@ -264,9 +264,12 @@ struct Launch {
 static double sweep_cost_us(const SolvePlan &S, const GroupPlan &g, uint32_t K) {
   const double n = (double)S.n, D = (double)std::max<uint32_t>(3u, (uint32_t)((S.inst->row_ptr[S.n] + S.n - 1) / S.n));
   const double metro = 0.0277 * n * g.G * (1.0 + 0.15 * (D - 3.0));
-  const double icm = g.P ? 0.0123 * n * ((g.P + kMaxPairs - 1) / kMaxPairs) : 0.0;
+  // ICM: overlap, seeds and growth (57 %) run in every member, the flip (43 %) is shared with one
+  // more meeting per chunk
+  const double chunks = g.P ? (double)((g.P + kMaxPairs - 1) / kMaxPairs) : 0.0;
+  const double icm = 0.0123 * n * chunks * (0.57 + 0.43 / K);
   const double levels = (double)S.inst->sched[S.P.sweep_order].n_classes;
-  return metro / K + icm + 8.0 + (K > 1 ? 4.0 * levels : 0.0);
+  return metro / K + icm + 8.0 + (K > 1 ? 4.0 * (levels + chunks) : 0.0);
 }
 
 // Teams need a group whose ICM scratch is in LDS (members keep private copies), no QUBO self
@@ -453,7 +456,9 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     ctr_off[i] = words;
     words += (kTeamCtrWords * 4u + 63u) / 64u * 8u;
     box_off[i] = words;
-    words += (size_t)items[i].S->n * items[i].g->G + (size_t)members[i] * items[i].g->R;
+    // level words, energy sums, then two flip buffers of block words and per-pair sums
+    const size_t nw = (size_t)items[i].S->n * items[i].g->G;
+    words += nw + (size_t)members[i] * items[i].g->R + 2u * (nw + (size_t)members[i] * kMaxPairs);
     words = (words + 7u) & ~(size_t)7u;
     ++Lc.teams;
   }
