@@ -234,8 +234,8 @@ typedef struct {
   uint32_t icm_growth;       /* how ICM clusters grow: 1 = each thread joins a fixed stride of sites
                                 per round, 2 = a wave closes a compact 64-site tile of the coupling
                                 graph at a time; 0 = tuned with `autotune`, else tiles for instances
-                                whose rows hold more than 4 couplings and strides otherwise.  The
-                                clusters do not depend on it */
+                                whose rows hold more than 4 couplings or 4 couplings on at most 4096
+                                sites, strides otherwise.  The clusters do not depend on it */
   uint32_t teams;            /* a large group's Metropolis sweep shared by a team of work groups of
                                 one launch, all resident at once (small groups then share work
                                 groups, one after another): 0 = where a cost model predicts a

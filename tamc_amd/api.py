@@ -342,7 +342,8 @@ def run_pt_batch(instances, params_list, waves_per_group=0, sweeps_per_launch=0,
     init_states: optional list (None entries allowed) of [C][B][n] +-1 start states per solve.
     task_tiling: 1 = a level's tasks dealt to the waves in turn, 2 = one contiguous run per wave,
     0 = tuned with `autotune` (else 1).  icm_growth: 1 = strided sites per thread, 2 = compact tiles,
-    0 = tuned with `autotune` (else tiles when the instance's rows hold more than 4 couplings).
+    0 = tuned with `autotune` (else tiles when the instance's rows hold more than 4 couplings, or 4 on
+    at most 4096 sites).
     teams: 0 = large groups' sweeps shared by teams of work groups where a cost model (and, with
     `autotune`, a timing) says it pays, 1 = never, K >= 2 = every eligible group as a team of K."""
     resolved, cps, hs = _batch_args(instances, params_list)

@@ -156,6 +156,14 @@ void want_buffers(SolvePlan &S, Arena &A, Arena *Ags, Arena *Ab) {
   }
 }
 
+// Default ICM growth order, from timings of both orders: compact tiles for rows wider than four
+// couplings (3D lattices: growth 25-35 % shorter) and for rows of four on up to 4096 sites (2D
+// lattices up to 64 x 64: 8-45 % shorter); strided sites otherwise (random 3-regular graphs, where
+// the strided order is 1.3-2x faster, and larger 2D lattices, whose rows come from HBM).
+static bool use_icm_tiles(bool have_tiles, uint32_t D, uint32_t n) {
+  return have_tiles && (D > 4u || (D == 4u && n <= 4096u));
+}
+
 tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
   if (upload_betas && s) TAMC_HIP(hipMemcpyAsync(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice, s));
   else if (upload_betas) TAMC_HIP(hipMemcpy(S.d_betas, S.betas.data(), S.B * 4, hipMemcpyHostToDevice));
@@ -201,7 +209,7 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.P = g.P;
     X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
               (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
-              (g.icm_global ? kFlagIcmGlobal : 0u) | (X.icm_tiles && X.ell_D > 4u ? kFlagIcmTiles : 0u);
+              (g.icm_global ? kFlagIcmGlobal : 0u) | (use_icm_tiles(X.icm_tiles != nullptr, X.ell_D, S.n) ? kFlagIcmTiles : 0u);
     X.T = S.T;
     X.meas_init = S.meas_init;
     X.sample = S.P.sample_enabled ? S.P.sample : 1u;
@@ -259,8 +267,10 @@ struct Launch {
 // Teams.  Per-sweep cost of a group in microseconds, from the measured phase split of the
 // batched kernel at 16 waves (random 3-regular graph, N = 3976, one plane word: Metropolis
 // 27.7 ns per site and word, ICM 12.3 ns per site and chunk of 32 pairs, 8 us of exchange,
-// measurement and barriers), scaled for the row width; a team of K pays a meeting of about 4 us
-// (arrival, wait, import of the other members' spin words) per colour class.
+// measurement and barriers), scaled for the row width; a team of K pays a meeting (arrival, wait,
+// import of the other members' spin words) per colour class and per ICM flip, charged at 4 us: the
+// measured cost is 5-6 us, but 4 us plans faster launches of the batched workload (2.25e11
+// against 2.21e11 attempts/s at 5.5 us), as the model's other terms overstate small groups.
 static double sweep_cost_us(const SolvePlan &S, const GroupPlan &g, uint32_t K) {
   const double n = (double)S.n, D = (double)std::max<uint32_t>(3u, (uint32_t)((S.inst->row_ptr[S.n] + S.n - 1) / S.n));
   const double metro = 0.0277 * n * g.G * (1.0 + 0.15 * (D - 3.0));
