@@ -167,8 +167,8 @@ struct Ctx {
   long long *dEa;
   unsigned long long *ticks;  // LDS phase timers [kPhaseSlots] and the last timestamp
   uint32_t *fscr;             // ICM labelling: [p] a site of pair p's partial cluster
-  unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [k][v] = replicas of
-                              // the pairs 4k + j with bit j of v set (one plane word per site)
+  unsigned long long *fmask;  // ICM flip masks by nibble of a cluster word: [g][k][v] = replicas, in
+                              // plane word g, of the pairs 4k + j with bit j of v set
   uint32_t n, G, R, B, P, nch, lo, W, wave, lane, tid, nthreads, key0, key1, flags, nblk;
   uint32_t members, member;
   bool icm_lds;
@@ -183,11 +183,9 @@ __device__ __forceinline__ void lap(const Ctx &c, int ph) {
   }
 }
 
-// Meeting point of a team (the work groups that share a group's sweep).  Each member's global
-// stores before the call are visible to every member after it.  Waits are bounded: a member that
-// waits longer than kTeamSpinLimit polls raises the team's abort flag, and once the flag is up no
-// member waits again, so the launch always ends (the host then reports the run as failed).  The
-// member's count of meetings is kept in misc[15] (misc[12..14] are the labelling's round flags).
+// Teams (the work groups that share a group's sweep) meet after each colour class and each ICM
+// flip.  The member's count of meetings is kept in misc[15] (misc[12..14] are the labelling's
+// round flags).  Waits are bounded by kTeamSpinLimit polls.
 constexpr uint32_t kTeamSpinLimit = 1u << 20;
 
 // The words the members exchange are written and read with agent-scope atomic stores and loads,
@@ -201,27 +199,77 @@ __device__ __forceinline__ unsigned long long box_load(const unsigned long long 
   return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void team_sync(const Ctx &c) {
-  const TAMC_AS4 PtGroup &X = *c.X;
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's box stores have completed
-  __syncthreads();
-  if (c.tid == 0u) {
-    uint32_t *ctr = X.team_ctr;
-    const uint32_t e = c.misc[15] + 1u;
-    c.misc[15] = e;
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t target = e * c.members;
-    uint32_t spins = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(ctr + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
-      if (++spins > kTeamSpinLimit) {
-        __hip_atomic_store(ctr + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+// Values cross between members as tagged slots: a 64-bit value takes two 64-bit slots, each holding
+// 32 bits of the value beside the 32-bit epoch of the meeting that published it (the members count
+// their meetings alike, from the count kept in team_ctr between launches).  A reader polls the
+// slots it needs until both carry the meeting's epoch, so a meeting costs one round trip to memory
+// and no counter.  A slot is published again only at a later meeting of the same kind, which the
+// publisher reaches only after every member has published at an intervening meeting, that is,
+// after every member has read the slot; ICM merges of consecutive chunks alternate between two
+// buffers for this reason.
+__device__ __forceinline__ void tag_put(unsigned long long *p, uint64_t v, uint32_t e) {
+  box_store(p, (v & 0xFFFFFFFFull) | ((uint64_t)e << 32));
+  box_store(p + 1, (v >> 32) | ((uint64_t)e << 32));
+}
+
+// The values published at meeting e in up to four tagged values (value j is read when bit j of
+// `want` is set), their loads issued together: values already published cost one round trip.
+// Waits are bounded: after kTeamSpinLimit polls the reader raises the team's abort flag, and once
+// the flag is up no reader waits again (the values are then meaningless and the host reports the
+// run as failed).
+__device__ __forceinline__ void tag_get4(const unsigned long long *const src[4], uint32_t want, uint32_t e,
+                                         uint32_t *abort_flag, uint64_t v[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = 0ull;
+  for (uint32_t spins = 0; want != 0u; ++spins) {
+    uint64_t a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = b[j] = 0ull;
+      if ((want >> j) & 1u) {
+        a[j] = box_load(src[j]);
+        b[j] = box_load(src[j] + 1);
       }
-      __builtin_amdgcn_s_sleep(2);
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (((want >> j) & 1u) && (uint32_t)(a[j] >> 32) == e && (uint32_t)(b[j] >> 32) == e) {
+        v[j] = (a[j] & 0xFFFFFFFFull) | (b[j] << 32);
+        want &= ~(1u << j);
+      }
+    }
+    if (want == 0u) return;
+    if ((spins & 15u) == 15u) {
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
+      if (spins > kTeamSpinLimit) {
+        __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+      }
+    }
+    __builtin_amdgcn_s_sleep(1);
   }
-  __syncthreads();
+}
+
+// Sum over the K members of the value each published at meeting e in sums[m * stride], in member
+// order, with the member's own value `mine` taken locally.
+__device__ __forceinline__ long long tag_sum(const unsigned long long *sums, size_t stride, uint32_t K,
+                                             uint32_t member, long long mine, uint32_t e, uint32_t *abort_flag) {
+  long long s = 0;
+  for (uint32_t m0 = 0; m0 < K; m0 += 4u) {
+    const unsigned long long *src[4];
+    uint32_t want = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+      src[j] = sums + 2u * (m0 + j) * stride;
+      if (m0 + j < K && m0 + j != member) want |= 1u << j;
+    }
+    uint64_t v[4];
+    tag_get4(src, want, e, abort_flag, v);
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (m0 + j < K) s += (m0 + j == member) ? mine : (long long)v[j];
+  }
+  return s;
 }
 
 // Spin bits of one site for replica indices chosen at run time.  With one 64-bit plane word per
@@ -338,9 +386,6 @@ template <int D, bool kLds>
 __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t P = c.P, n = c.n, G = c.G;
-  // replica indices of every pair, lane p holding pair p
-  const uint32_t pa_l = (c.lane < P) ? c.pair_a[c.lane] : 0u;
-  const uint32_t pb_l = (c.lane < P) ? c.pair_b[c.lane] : 0u;
   constexpr int kD = D > 0 ? D : 1;
   // the terms of the pairs in `pmask` for the sites of block blk
   auto terms = [&](uint32_t blk, uint32_t cwi, uint32_t pmask) {
@@ -414,11 +459,7 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
     }
     for (uint32_t g = 0; g < G; ++g) {
       uint64_t mg = 0ull;
-      for (uint32_t p = 0; p < P; ++p) {
-        const uint32_t a = __builtin_amdgcn_readlane(pa_l, p), b = __builtin_amdgcn_readlane(pb_l, p);
-        const uint64_t bits = ((a >> 6) == g ? 1ull << (a & 63u) : 0ull) | ((b >> 6) == g ? 1ull << (b & 63u) : 0ull);
-        if ((cwi >> p) & 1u) mg |= bits;
-      }
+      for (uint32_t k = 0; 4u * k < P; ++k) mg |= c.fmask[(g * (kMaxPairs / 4u) + k) * 16u + ((cwi >> (4u * k)) & 15u)];
       if (mg) {
         c.sp[(i_lane * G + g) * 2u] ^= (uint32_t)mg;
         c.sp[(i_lane * G + g) * 2u + 1u] ^= (uint32_t)(mg >> 32);
@@ -457,40 +498,37 @@ __device__ __forceinline__ void icm_flip(Ctx &c, const IcmMem<kLds> &M) {
       if (ballot(cwi != 0u)) flip_block(blk, cwi);
     }
   }
+  // In a team the member flips the other members' blocks too, without their terms: every member
+  // holds every cluster, and a flip of pair q touches replicas a_q and b_q only, so it changes no
+  // bit that a term of another pair reads (and the member's own sites are flipped by itself).
+  if (K > 1u) {
+    for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
+      if (blk % K == c.member) continue;
+      const uint32_t i_lane = blk * 64u + c.lane;
+      const uint32_t cwi = (i_lane < n) ? M.cw[i_lane] : 0u;
+      if (ballot(cwi != 0u)) flip_block(blk, cwi);
+    }
+  }
 }
 
-// In a team every member flips its own 64-site blocks and sums their terms; the members then
-// publish the spin words of their blocks and their per-pair sums, meet, and take the others'.
-// Two buffers alternate by ICM chunk, so a member that runs ahead into the next chunk does not
-// overwrite words another member is still reading.
+// In a team every member sums the terms of its own 64-site blocks (and flips every block): the
+// members exchange only their per-pair sums.  Two buffers alternate by ICM chunk, so a member that
+// runs ahead into the next chunk does not overwrite sums another member is still reading.
 __device__ __forceinline__ void icm_team_merge(Ctx &c) {
   const TAMC_AS4 PtGroup &X = *c.X;
-  const uint32_t K = c.members, G = c.G, n = c.n, P = c.P;
-  const size_t words = (size_t)n * G;
+  const uint32_t K = c.members, P = c.P;
   const uint32_t par = ((c.lo - X.lo) / kMaxPairs) & 1u;
-  unsigned long long *F = X.team_box + words + (size_t)K * c.R + par * (words + (size_t)K * kMaxPairs);
-  unsigned long long *Dp = F + words;
+  unsigned long long *Dp = X.team_box + team_pair_slot(c.n, c.G, c.R, K, par);
+  uint32_t *abort_flag = X.team_ctr + 1;
   __syncthreads();
-  const uint32_t nbm = c.nblk > c.member ? (c.nblk - c.member + K - 1u) / K : 0u;
-  for (uint32_t idx = c.tid; idx < nbm * 64u * G; idx += c.nthreads) {
-    const uint32_t j = idx / (64u * G), rem = idx - j * 64u * G;
-    const uint32_t i = (c.member + j * K) * 64u + rem / G, g = rem % G;
-    if (i < n) box_store(F + (size_t)i * G + g, *reinterpret_cast<const uint64_t *>(c.sp + (i * G + g) * 2u));
-  }
+  const uint32_t e = c.misc[15] + 1u;
   for (uint32_t p = c.tid; p < P; p += c.nthreads)
-    box_store(Dp + (size_t)c.member * kMaxPairs + p, (unsigned long long)c.dEa[p]);
-  team_sync(c);
-  for (uint32_t idx = c.tid; idx < n * G; idx += c.nthreads) {
-    const uint32_t i = idx / G;
-    if ((i / 64u) % K == c.member) continue;
-    *reinterpret_cast<uint64_t *>(c.sp + idx * 2u) = box_load(F + idx);
-  }
-  for (uint32_t p = c.tid; p < P; p += c.nthreads) {
-    long long s = 0;
-    for (uint32_t m = 0; m < K; ++m) s += (long long)box_load(Dp + (size_t)m * kMaxPairs + p);
-    c.dEa[p] = s;
-  }
+    tag_put(Dp + 2u * ((size_t)c.member * kMaxPairs + p), (unsigned long long)c.dEa[p], e);
+  // then sums all the members' terms in member order, so that every member gets the same totals
+  for (uint32_t p = c.tid; p < P; p += c.nthreads)
+    c.dEa[p] = tag_sum(Dp + 2u * p, kMaxPairs, K, c.member, c.dEa[p], e, abort_flag);
   __syncthreads();
+  if (c.tid == 0u) c.misc[15] = e;
 }
 
 // ICM step (c) for the pairs whose clusters still grow after kLabelAfterPasses passes (long
@@ -948,14 +986,20 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     c.dEa[c.tid] = 0;
   }
   __syncthreads();
-  if (G == 1u && c.tid < 16u * ((P + 3u) / 4u)) {
-    const uint32_t k = c.tid >> 4, v = c.tid & 15u;
+  // flip masks: entry (g, k, v) holds the replicas in plane word g of the pairs 4k + j with bit j
+  // of v set
+  for (uint32_t idx = c.tid; idx < G * (kMaxPairs / 4u) * 16u; idx += c.nthreads) {
+    const uint32_t g = idx / ((kMaxPairs / 4u) * 16u), k = (idx >> 4) % (kMaxPairs / 4u), v = idx & 15u;
+    if (4u * k >= P) continue;
     unsigned long long m = 0ull;
     for (uint32_t j = 0; j < 4u; ++j) {
       const uint32_t p = 4u * k + j;
-      if (p < P && ((v >> j) & 1u)) m |= (1ull << c.pair_a[p]) | (1ull << c.pair_b[p]);
+      if (p >= P || !((v >> j) & 1u)) continue;
+      const uint32_t a = c.pair_a[p], b = c.pair_b[p];
+      if ((a >> 6) == g) m |= 1ull << (a & 63u);
+      if ((b >> 6) == g) m |= 1ull << (b & 63u);
     }
-    c.fmask[c.tid] = m;  // read only after the growth loop's barriers
+    c.fmask[idx] = m;  // read only after the growth loop's barriers
   }
   // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; blk[p][b] =
   //     number of such sites in 64-site block b
@@ -1288,8 +1332,12 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   uint32_t a = c4(X.lvl_off)[0];
   const bool blocks = (c.flags & kFlagTileBlocks) != 0u;
   const uint32_t K = kTeam ? c.members : 1u;
+  // team meetings: one per level, numbered on from the member's count
+  uint32_t e = kTeam ? c.misc[15] : 0u;
+  uint32_t *abort_flag = kTeam ? X.team_ctr + 1 : nullptr;
   for (uint32_t l = 0; l < nl; ++l) {
     const uint32_t b = c4(X.lvl_off)[l + 1];
+    if (K > 1u) ++e;
     // tiling of the level's tasks [a, b) over the waves: dealt in turn, or one contiguous run each.
     // In a team the member takes the RNG groups g == member (mod members) of the level, dealt in
     // turn: index j stands for word j % G of the level's (j / G)-th such group.
@@ -1398,7 +1446,7 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
         // the site's new plane word: the lanes' old bits with the accepted ones flipped
         const uint64_t nw = ballot(bi[s] != 0u) ^ accm;
         if (c.lane == 0u) *reinterpret_cast<uint64_t *>(c.sp + (i * G + g) * 2u) = nw;
-        if (K > 1u && c.lane == 0u) box_store(X.team_box + (size_t)i * G + g, nw);
+        if (K > 1u && c.lane == 0u) tag_put(X.team_box + 2u * ((size_t)i * G + g), nw, e);
         const bool acc_ok = __builtin_amdgcn_inverse_ballot_w64(accm);
         if (acc_ok) a_g += dd[s];
         if (self) {
@@ -1412,25 +1460,40 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
     if (K > 1u) {
       // after the last level, this member's share of the accepted energy goes out with its spin
       // words, so that one meeting serves both
-      unsigned long long *sums = X.team_box + (size_t)c.n * G;
+      unsigned long long *sums = X.team_box + team_sums_slot(c.n, G);
       if (l + 1u == nl) {
         __syncthreads();
         for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
           const uint32_t g = r >> 6, ln = r & 63u;
           long long s = 0;
           for (uint32_t w = 0; w < c.W; ++w) s += c.partial[(w * G + g) * 64u + ln];
-          box_store(sums + (size_t)c.member * c.R + r, (unsigned long long)s);
+          tag_put(sums + 2u * ((size_t)c.member * c.R + r), (unsigned long long)s, e);
         }
       }
-      // take the other members' new spin words of this level
-      team_sync(c);
+      // take the other members' new spin words of this level, four per thread at a time
       const uint32_t items = (gb - ga) * 4u * G;
-      for (uint32_t it = c.tid; it < items; it += c.nthreads) {
-        const uint32_t q = ga + it / (4u * G);
-        if (q % K == c.member) continue;
-        const uint32_t site = c4(X.grp)[q * 4u + (it / G) % 4u], g = it % G;
-        if (site != kNone)
-          *reinterpret_cast<uint64_t *>(c.sp + (site * G + g) * 2u) = box_load(X.team_box + (size_t)site * G + g);
+      for (uint32_t it0 = c.tid; it0 < items; it0 += 4u * c.nthreads) {
+        const unsigned long long *src[4];
+        uint32_t dst[4], want = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j) {
+          const uint32_t it = it0 + j * c.nthreads;
+          src[j] = X.team_box;
+          dst[j] = 0;
+          if (it >= items) continue;
+          const uint32_t q = ga + it / (4u * G);
+          if (q % K == c.member) continue;
+          const uint32_t site = c4(X.grp)[q * 4u + (it / G) % 4u], g = it % G;
+          if (site == kNone) continue;
+          src[j] = X.team_box + 2u * ((size_t)site * G + g);
+          dst[j] = (site * G + g) * 2u;
+          want |= 1u << j;
+        }
+        uint64_t v[4];
+        tag_get4(src, want, e, abort_flag, v);
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j)
+          if ((want >> j) & 1u) *reinterpret_cast<uint64_t *>(c.sp + dst[j]) = v[j];
       }
     }
     a = b;
@@ -1439,13 +1502,15 @@ __device__ __forceinline__ void sweep_phase_d(Ctx &c, uint32_t t) {
   __syncthreads();
   if (K > 1u) {
     // every member adds every member's share, in member order: the same energies everywhere
-    const unsigned long long *sums = X.team_box + (size_t)c.n * G;
+    const unsigned long long *sums = X.team_box + team_sums_slot(c.n, G);
     for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
-      long long s = 0;
-      for (uint32_t m = 0; m < K; ++m) s += (long long)box_load(sums + (size_t)m * c.R + r);
-      c.energy[r] = add_fix(c.energy[r], s);
+      const uint32_t g = r >> 6, ln = r & 63u;
+      long long mine = 0;
+      for (uint32_t w = 0; w < c.W; ++w) mine += c.partial[(w * G + g) * 64u + ln];
+      c.energy[r] = add_fix(c.energy[r], tag_sum(sums + 2u * r, c.R, K, c.member, mine, e, abort_flag));
     }
     __syncthreads();
+    if (c.tid == 0u) c.misc[15] = e;
     return;
   }
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {

@@ -467,8 +467,7 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     words += (kTeamCtrWords * 4u + 63u) / 64u * 8u;
     box_off[i] = words;
     // level words, energy sums, then two flip buffers of block words and per-pair sums
-    const size_t nw = (size_t)items[i].S->n * items[i].g->G;
-    words += nw + (size_t)members[i] * items[i].g->R + 2u * (nw + (size_t)members[i] * kMaxPairs);
+    words += team_box_slots(items[i].S->n, items[i].g->G, items[i].g->R, members[i]);
     words = (words + 7u) & ~(size_t)7u;
     ++Lc.teams;
   }

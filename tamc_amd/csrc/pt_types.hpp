@@ -100,17 +100,29 @@ struct PtGroup {
   // ---- teams: a group whose Metropolis sweep is shared by `members` work groups of one launch,
   //      all resident at once.  Every member holds the whole state in its LDS and runs ICM and
   //      the exchange itself (the same inputs give the same results); member m updates the RNG
-  //      groups g with g % members == m, publishes the new spin words (and, after the last colour
-  //      class, its share of the accepted energy) through team_box, and meets the others at a
-  //      counter after each colour class.  Member 0 alone writes measurements and the final
-  //      state. ----
+  //      groups g with g % members == m and publishes the new spin words (and, after the last
+  //      colour class, its share of the accepted energy) in team_box, tagged with the number of the
+  //      meeting; the others read them when the tags match.  In ICM every member flips every
+  //      cluster and sums the terms of its own blocks; the members exchange the sums.  Member 0
+  //      alone writes measurements and the final state. ----
   uint32_t members, member;  // 1, 0 for a group run by one work group
-  uint32_t *team_ctr;        // [kTeamCtrWords]: arrivals, abort flag, -, epochs of member 0, 1, ...
-  unsigned long long *team_box;  // [n * G] spin words, then [members][R] accepted-energy sums
+  uint32_t *team_ctr;        // [kTeamCtrWords]: -, abort flag, -, meeting counts of member 0, 1, ...
+  unsigned long long *team_box;  // tagged values (team_box_slots)
 };
 
 constexpr uint32_t kMaxMembers = 16;
 constexpr uint32_t kTeamCtrWords = 3 + kMaxMembers;
+
+// Team box, in 64-bit slots; every value takes two slots (32 bits of it beside the epoch of the
+// meeting that published it): the spin words [n * G], the members' accepted-energy sums
+// [members][R], then two buffers of the members' ICM per-pair sums [members][kMaxPairs].
+__host__ __device__ inline size_t team_sums_slot(uint32_t n, uint32_t G) { return 2u * (size_t)n * G; }
+__host__ __device__ inline size_t team_pair_slot(uint32_t n, uint32_t G, uint32_t R, uint32_t K, uint32_t par) {
+  return team_sums_slot(n, G) + 2u * (size_t)K * R + par * 2u * (size_t)K * kMaxPairs;
+}
+__host__ __device__ inline size_t team_box_slots(uint32_t n, uint32_t G, uint32_t R, uint32_t K) {
+  return team_pair_slot(n, G, R, K, 2u);
+}
 
 struct LdsLayout {
   uint32_t spins, energy, beta_of, perm, tags, nacc, hist, rt, pair_a, pair_b, pair_of, seed_blk, misc, ticks, fmask, fscr, partial, dE, drift,
@@ -158,8 +170,8 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(16u * 4u);
   L.ticks = o;  // phase timers [kPhaseSlots] + last timestamp
   o += lds_align((kPhaseSlots + 1u) * 8u);
-  L.fmask = o;  // ICM flip masks [kMaxPairs / 4][16] (groups with one plane word per site)
-  o += lds_align(P > 0 ? kMaxPairs / 4u * 16u * 8u : 8u);
+  L.fmask = o;  // ICM flip masks [G][kMaxPairs / 4][16]
+  o += lds_align(P > 0 ? G * (kMaxPairs / 4u * 16u * 8u) : 8u);
   L.fscr = o;  // ICM cluster labelling: a site of each straggling pair's partial cluster
   o += lds_align(P > 0 ? kMaxPairs * 4u : 4u);
   L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G)
