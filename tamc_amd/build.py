@@ -33,7 +33,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["instance.hip", "pt_kernels.hip", "pt_runtime.hip", "lockstep.hip", "elementary.hip", "sa.hip",
+SOURCES = ["pt_kernels.hip", "pt_kernels_any.hip", "instance.hip", "pt_runtime.hip", "lockstep.hip", "elementary.hip", "sa.hip",
            "writers.hip", "server.hip", "gla.hip", "pt_opt.hip", "percolation.hip"]
 
 

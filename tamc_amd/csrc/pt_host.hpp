@@ -15,7 +15,8 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
                                           const uint32_t *const *init_bits, hipStream_t s);
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
-                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs);
+                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs,
+                                            uint32_t spins_global);
 extern "C" hipError_t tamc_pt_sweep_capacity(uint32_t waves, uint32_t lds_bytes, uint32_t *capacity);
 
 namespace tamc {
@@ -32,6 +33,7 @@ inline T *host_dup(const std::vector<T> &v) {
 struct GroupPlan {
   uint32_t chain0, nch, R, G, P;
   bool icm_global = false;
+  bool spins_global = false;  // the spin planes stay in global memory (implies icm_global)
   PtGroup g{};  // device pointers filled at allocation
   std::vector<uint32_t> init_bits;
 };

@@ -86,6 +86,7 @@ __device__ __forceinline__ uint32_t sbit(const uint32_t *sp, uint32_t G, uint32_
   return (sp[(i * G + (r >> 6)) * 2u + ((r >> 5) & 1u)] >> (r & 31u)) & 1u;
 }
 
+#ifndef TAMC_SWEEP_KERNEL_ANY
 // --------------------------------------------------------------------------------------------
 // Device-side initialisation of a group: bit planes from init states or Philox, energies by
 // energy_ref in the reference's order (src/ising.rs:438-454), identity permutation, fresh tags.
@@ -150,6 +151,8 @@ __global__ void pt_init_kernel(const PtGroup *groups, const uint32_t *const *ini
     X.icm_moves[0] = 0ull;
   }
 }
+
+#endif  // TAMC_SWEEP_KERNEL_ANY
 
 // --------------------------------------------------------------------------------------------
 struct Ctx {
@@ -968,7 +971,7 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
 
 // 1. Houdayer ICM for all pairs of the group at once (src/pt.rs:31-77).  Pair p couples slot lo+p of
 // the two chains; bit p of the per-site words qw/cw belongs to pair p.
-template <bool kLds>
+template <bool kLds, bool kSpinsGlobal = false>
 __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   const TAMC_AS4 PtGroup &X = *c.X;
   using U32 = typename IcmMem<kLds>::U32;
@@ -1090,7 +1093,9 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   }
   lap(c, kPhIcmSeed);
   __syncthreads();
-  if constexpr (kLds) {
+  if constexpr (kLds || kSpinsGlobal) {
+    // (with the spin planes in global memory there is no LDS to borrow: the clusters grow in the
+    // global scratch)
     icm_grow(c, M);
   } else {
     // the spins are not read while clusters grow: park the planes in the group's spin array and
@@ -1118,7 +1123,9 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   }
   lap(c, kPhIcmGrow);
   // (d) swap the cluster spins of both replicas (= flip both) and sum the energy change
-  switch (X.ell_D) {
+  //     (the fixed-width terms read the 16-bit neighbour lists, which instances of 65,536 sites or
+  //     more do not have: those walk the coupling rows)
+  switch (c.nbr ? X.ell_D : 0u) {
     case 3u: icm_flip<3>(c, M); break;
     case 4u: icm_flip<4>(c, M); break;
     case 6u: icm_flip<6>(c, M); break;
@@ -1664,6 +1671,7 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
 // pointer passed through an empty asm statement: that keeps the compiler from holding dozens of
 // loop-invariant scalars live across all phases, which it would otherwise spill to VGPR lanes and
 // read back with v_readlane inside the hot loops.
+template <bool kAny>
 __device__ __forceinline__ Ctx make_ctx(const PtGroup *group, uint32_t icm_lds, uint32_t lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   const TAMC_AS4 PtGroup &X = *c4(group);
@@ -1689,9 +1697,12 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *group, uint32_t icm_lds, 
   c.key1 = (uint32_t)(X.seed >> 32);
   const bool use_lds_icm = icm_lds != 0u && !(X.flags & kFlagIcmGlobal);
   c.icm_lds = use_lds_icm;
-  const LdsLayout L = with_nbr(make_layout(c.n, c.G, c.R, c.B, c.P, c.W, use_lds_icm), c.n,
+  // kAny: the spin planes may stay in global memory (kFlagSpinsGlobal), so c.sp is a generic
+  // pointer; otherwise they are in LDS and every access is an LDS instruction
+  const bool spins_global = kAny && (X.flags & kFlagSpinsGlobal) != 0u;
+  const LdsLayout L = with_nbr(make_layout(c.n, c.G, c.R, c.B, c.P, c.W, use_lds_icm, !spins_global), c.n,
                                X.ell_col16 ? X.ell_D : 0u, lds_bytes);
-  c.sp = (uint32_t *)(smem + L.spins);
+  c.sp = spins_global ? X.spins : (uint32_t *)(smem + L.spins);
   c.energy = (float *)(smem + L.energy);
   c.beta_of = (float *)(smem + L.beta_of);
   c.perm = (uint32_t *)(smem + L.perm);
@@ -1735,12 +1746,14 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *group, uint32_t icm_lds, 
 }
 
 // One group (or one team member's copy of it): load the persistent state into LDS, run sweeps
-// [t0, t1), store the state back.
+// [t0, t1), store the state back.  kAny: the group's spin planes may stay in global memory.
+template <bool kAny>
 __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uint32_t t1, uint32_t icm_lds,
                                           uint32_t lds_bytes, unsigned long long launch_start,
                                           unsigned long long *launch_ticks) {
-  Ctx c = make_ctx(group, icm_lds, lds_bytes);
+  Ctx c = make_ctx<kAny>(group, icm_lds, lds_bytes);
   const TAMC_AS4 PtGroup &X = *c.X;
+  const bool spins_lds = !(kAny && (c.flags & kFlagSpinsGlobal));
   const bool lead = c.member == 0u;  // the member that writes outputs and the final state
   if (c.P > 0 && c.nbr_in_lds) {
     uint16_t *nb = const_cast<uint16_t *>(c.nbr);
@@ -1753,7 +1766,8 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
   }
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
-  for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
+  if (spins_lds)
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
     c.energy[r] = X.energy[r];
     c.drift[r] = X.drift[r];
@@ -1777,12 +1791,13 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
     const PtGroup *gp = group;
     uint32_t il = icm_lds, lb = lds_bytes;
     asm volatile("" : "+s"(gp), "+s"(il), "+s"(lb));
-    Ctx cs = make_ctx(gp, il, lb);
+    Ctx cs = make_ctx<kAny>(gp, il, lb);
     if ((cs.flags & kFlagIcm) && cs.P > 0) {
       const uint32_t Pt = cs.P;
       cs.lo += ck * kMaxPairs;
       cs.P = icm_chunk(Pt - ck * kMaxPairs);
       if (cs.icm_lds) icm_phase<true>(cs, t);
+      else if (kAny && !spins_lds) icm_phase<false, true>(cs, t);
       else icm_phase<false>(cs, t);
       if ((++ck) * kMaxPairs < Pt) continue;
     }
@@ -1806,7 +1821,8 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
   if (c.tid == 0) X.icm_moves[0] += c.moves[0];
   if (c.tid == 0 && launch_ticks) launch_ticks[blockIdx.x] = wall_clock64() - launch_start;
   // store persistent state
-  for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
+  if (spins_lds)
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
   for (uint32_t r = c.tid; r < c.R; r += c.nthreads) {
     X.energy[r] = c.energy[r];
     X.drift[r] = c.drift[r];
@@ -1824,9 +1840,10 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
 // Work group b runs the groups jobs[b] .. jobs[b + 1] - 1 one after another (without a job table,
 // group b alone).  The members of a team are in different work groups of one launch, all resident
 // at once, and come first in their work groups' job lists.
-__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
-                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
-                                                        unsigned long long *launch_ticks, const uint32_t *jobs) {
+template <bool kAny>
+__device__ __forceinline__ void sweep_kernel_body(const PtGroup *groups, uint32_t t0, uint32_t t1, uint32_t icm_lds,
+                                                  uint32_t lds_bytes, const uint32_t *ranges,
+                                                  unsigned long long *launch_ticks, const uint32_t *jobs) {
   if (ranges) {
     t0 = c4(ranges)[2u * blockIdx.x];
     t1 = c4(ranges)[2u * blockIdx.x + 1u];
@@ -1835,12 +1852,56 @@ __global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, u
   const uint32_t j0 = jobs ? c4(jobs)[blockIdx.x] : blockIdx.x;
   const uint32_t j1 = jobs ? c4(jobs)[blockIdx.x + 1u] : blockIdx.x + 1u;
   for (uint32_t j = j0; j < j1; ++j) {
-    run_group(groups + j, t0, t1, icm_lds, lds_bytes, launch_start, launch_ticks);
+    run_group<kAny>(groups + j, t0, t1, icm_lds, lds_bytes, launch_start, launch_ticks);
     __syncthreads();
   }
 }
 
+#ifndef TAMC_SWEEP_KERNEL_ANY
+__global__ __launch_bounds__(1024) void pt_sweep_kernel(const PtGroup *groups, uint32_t t0, uint32_t t1,
+                                                        uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
+                                                        unsigned long long *launch_ticks, const uint32_t *jobs) {
+  sweep_kernel_body<false>(groups, t0, t1, icm_lds, lds_bytes, ranges, launch_ticks, jobs);
+}
+#else
+// The same for launches holding a group whose spin planes stay in global memory (a group too large
+// for LDS): spin accesses go through generic pointers.  Built from pt_kernels_any.hip.
+__global__ __launch_bounds__(1024) void pt_sweep_kernel_any(const PtGroup *groups, uint32_t t0, uint32_t t1,
+                                                            uint32_t icm_lds, uint32_t lds_bytes, const uint32_t *ranges,
+                                                            unsigned long long *launch_ticks, const uint32_t *jobs) {
+  sweep_kernel_body<true>(groups, t0, t1, icm_lds, lds_bytes, ranges, launch_ticks, jobs);
+}
+#endif
+
 }  // namespace tamc
+
+#ifdef TAMC_SWEEP_KERNEL_ANY
+extern "C" hipError_t tamc_launch_pt_sweeps_any(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
+                                                uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds,
+                                                const uint32_t *ranges, unsigned long long *launch_ticks, hipStream_t s,
+                                                const uint32_t *jobs) {
+  // dynamic LDS above 64 KiB must be requested explicitly; the attribute is raised to the largest
+  // size any launch has used (callers may launch from several threads)
+  static std::mutex mu;
+  static uint32_t attr_bytes = 65536u;
+  if (lds_bytes > 65536u) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (lds_bytes > attr_bytes) {
+      const hipError_t e = hipFuncSetAttribute((const void *)tamc::pt_sweep_kernel_any,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      if (e != hipSuccess) return e;
+      attr_bytes = lds_bytes;
+    }
+  }
+  hipLaunchKernelGGL(tamc::pt_sweep_kernel_any, dim3(ngroups), dim3(waves * 64u), lds_bytes, s, groups, t0, t1,
+                     icm_lds, lds_bytes, ranges, launch_ticks, jobs);
+  return hipGetLastError();
+}
+#else
+extern "C" hipError_t tamc_launch_pt_sweeps_any(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
+                                                uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds,
+                                                const uint32_t *ranges, unsigned long long *launch_ticks, hipStream_t s,
+                                                const uint32_t *jobs);
 
 // ------------------------------------------------------------------------ launch wrappers ----
 // Work groups of pt_sweep_kernel that can be resident at once on the device (a launch with teams
@@ -1867,7 +1928,10 @@ extern "C" hipError_t tamc_launch_pt_init(const tamc::PtGroup *groups, uint32_t 
 
 extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_t ngroups, uint32_t t0, uint32_t t1,
                                             uint32_t waves, uint32_t lds_bytes, uint32_t icm_lds, const uint32_t *ranges,
-                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs) {
+                                            unsigned long long *launch_ticks, hipStream_t s, const uint32_t *jobs,
+                                            uint32_t spins_global) {
+  if (spins_global)
+    return tamc_launch_pt_sweeps_any(groups, ngroups, t0, t1, waves, lds_bytes, icm_lds, ranges, launch_ticks, s, jobs);
   // dynamic LDS above 64 KiB must be requested explicitly; the attribute is raised to the largest
   // size any launch has used (callers may launch from several threads)
   static std::mutex mu;
@@ -1890,3 +1954,4 @@ extern "C" hipError_t tamc_launch_pt_sweeps(const tamc::PtGroup *groups, uint32_
                      lds_bytes, ranges, launch_ticks, jobs);
   return hipGetLastError();
 }
+#endif  // TAMC_SWEEP_KERNEL_ANY

@@ -69,11 +69,16 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
       return fail(TAMC_E_UNSUPPORTED, "a chain pair may hold at most 256 replicas (num_betas <= 128)");
     g.G = (g.R + 63) / 64;
     g.P = (P->icm && g.nch == 2 && P->lo_beta_idx < S.B) ? S.B - P->lo_beta_idx : 0;
+    // the group's state in LDS when it fits; else the ICM scratch, then also the spin planes, in
+    // global memory
     const LdsLayout L = make_layout(S.n, g.G, g.R, S.B, g.P, waves, true);
     if (L.total > kLdsLimit) {
-      const LdsLayout L2 = make_layout(S.n, g.G, g.R, S.B, g.P, waves, false);
-      if (L2.total > kLdsLimit) return fail(TAMC_E_UNSUPPORTED, "instance too large for the LDS-resident kernel");
       g.icm_global = true;
+      if (make_layout(S.n, g.G, g.R, S.B, g.P, waves, false).total > kLdsLimit) {
+        g.spins_global = true;
+        if (make_layout(S.n, g.G, g.R, S.B, g.P, waves, false, false).total > kLdsLimit)
+          return fail(TAMC_E_UNSUPPORTED, "a chain pair's ladder state does not fit in LDS");
+      }
     }
     if (init_states) {
       // bit planes [n][G][2]: bit l of half h of word g <=> replica 64g+32h+l is -1
@@ -119,7 +124,7 @@ tamc_status upload_resume(const SolvePlan &S) {
 }
 
 uint32_t lds_bytes_for(const SolvePlan &S, const GroupPlan &g, uint32_t waves) {
-  return with_nbr(make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global), S.n,
+  return with_nbr(make_layout(S.n, g.G, g.R, S.B, g.P, waves, !g.icm_global, !g.spins_global), S.n,
                   g.g.ell_col16 ? g.g.ell_D : 0u, kLdsLimit)
       .total;
 }
@@ -209,7 +214,7 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.P = g.P;
     X.flags = (S.P.icm ? kFlagIcm : 0u) | (S.P.threads > 1 ? kFlagThreaded : 0u) |
               (S.P.sample_enabled ? kFlagSample : 0u) | (S.P.sample_states_enabled ? kFlagSampleStates : 0u) |
-              (g.icm_global ? kFlagIcmGlobal : 0u) | (use_icm_tiles(X.icm_tiles != nullptr, X.ell_D, S.n) ? kFlagIcmTiles : 0u);
+              (g.icm_global ? kFlagIcmGlobal : 0u) | (g.spins_global ? kFlagSpinsGlobal : 0u) | (use_icm_tiles(X.icm_tiles != nullptr, X.ell_D, S.n) ? kFlagIcmTiles : 0u);
     X.T = S.T;
     X.meas_init = S.meas_init;
     X.sample = S.P.sample_enabled ? S.P.sample : 1u;
@@ -252,6 +257,7 @@ struct Launch {
   DevBuf d_groups, d_init_ptrs;
   std::vector<DevBuf *> init_bufs;
   uint32_t lds = 0, icm_lds = 1, max_T = 0;
+  uint32_t spins_global = 0;         // some group keeps its spin planes in global memory
   // with teams: the sweep launch's descriptors (one per team member or group), grouped by work
   // group through `jobs` (work group b runs desc[jobs[b]] .. desc[jobs[b + 1] - 1])
   std::vector<PtGroup> desc;
@@ -396,6 +402,7 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
       items.push_back({&S, &g, (double)S.n * g.R * S.T});
       any_init |= !g.init_bits.empty();
       Lc.lds = std::max(Lc.lds, lds_bytes_for(S, g, waves));
+      Lc.spins_global |= g.spins_global ? 1u : 0u;
       Lc.max_T = std::max(Lc.max_T, S.T);
     }
   std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
@@ -720,13 +727,13 @@ static tamc_status run_solves(std::vector<SolvePlan> &solves, const RunOptions &
   if (graph) {
     TAMC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      (void)tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs);
+      (void)tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs, Lc.spins_global);
     TAMC_HIP(hipStreamEndCapture(s, &hgraph));
     TAMC_HIP(hipGraphInstantiate(&gexec, hgraph, nullptr, nullptr, 0));
     TAMC_HIP(hipGraphLaunch(gexec, s));
   } else {
     for (uint32_t t0 = 0; t0 < Lc.max_T; t0 += spl, ++launches)
-      TAMC_HIP(tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs));
+      TAMC_HIP(tamc_launch_pt_sweeps(lg, lng, t0, t0 + spl, opt.waves, Lc.lds, Lc.icm_lds, nullptr, nullptr, s, ljobs, Lc.spins_global));
   }
   TAMC_HIP(hipEventRecord(e1, s));
   TAMC_HIP(hipEventSynchronize(e1));

@@ -32,6 +32,8 @@ enum : uint32_t {
   kFlagTileBlocks = 64u,     // sweep tiling: each wave takes one contiguous run of a level's tasks
                              // (else the tasks are dealt to the waves in turn)
   kFlagIcmTiles = 128u,      // ICM cluster growth over compact tiles of the coupling graph
+  kFlagSpinsGlobal = 256u,   // the spin planes stay in the group's global state array (group too
+                             // large for LDS; implies kFlagIcmGlobal)
 };
 
 struct PtGroup {
@@ -135,14 +137,15 @@ __host__ __device__ inline uint32_t lds_align(uint32_t x) { return (x + 15u) & ~
 // overlap and cluster words); the number of pairs one chunk holds
 __host__ __device__ inline uint32_t icm_chunk(uint32_t P) { return P < kMaxPairs ? P : kMaxPairs; }
 
-// icm_lds: ICM arrays (q words, cluster words, per-block counts) are placed in LDS.
+// icm_lds: ICM arrays (q words, cluster words, per-block counts) are placed in LDS; spins_lds: so
+// are the spin planes.
 __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_t R, uint32_t B, uint32_t P,
-                                                 uint32_t W, bool icm_lds) {
+                                                 uint32_t W, bool icm_lds, bool spins_lds = true) {
   LdsLayout L;
   uint32_t o = 0;
   const uint32_t nblk = (n + 63u) / 64u;
-  L.spins = o;
-  o += lds_align(n * G * 8u);
+  L.spins = spins_lds ? o : kNone;
+  if (spins_lds) o += lds_align(n * G * 8u);
   L.energy = o;
   o += lds_align(R * 4u);
   L.beta_of = o;

@@ -532,7 +532,7 @@ struct tamc_server {
       for (auto &bo : offsets[s]) {
         Bucket *b = bo.first;
         TAMC_HIP(tamc_launch_pt_sweeps(d_groups[s] + bo.second, (uint32_t)b->members.size(), 0, 0, b->waves, b->lds, 1u,
-                                       d_ranges[s] + 2 * bo.second, d_ticks[s] + bo.second, streams[s], nullptr));
+                                       d_ranges[s] + 2 * bo.second, d_ticks[s] + bo.second, streams[s], nullptr, 0u));
       }
       if (sa_here) {
         TAMC_HIP(hipEventRecord(ev_sa0, streams[s]));
@@ -776,6 +776,10 @@ tamc_status tamc_server_submit(tamc_server *srv, const tamc_instance *inst, cons
   r->waves = srv->opt.waves_per_group ? srv->opt.waves_per_group : (inst ? default_waves(inst) : 4u);
   tamc_status s = plan_solve(r->S, inst, params, init_states, r->waves);
   if (s) return s;
+  for (auto &g : r->S.groups)
+    if (g.spins_global)
+      return fail(TAMC_E_UNSUPPORTED, "the server runs chain pairs whose spin planes fit in LDS "
+                                      "(about 19,000 sites at 64 replicas); use tamc_pt_run for larger instances");
   if (r->S.groups.size() > srv->max_groups)
     return fail(TAMC_E_UNSUPPORTED, "request has more chain pairs than the server's work-group slots");
   r->S.init_states = nullptr;  // copied into the groups' init_bits

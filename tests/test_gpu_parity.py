@@ -159,6 +159,19 @@ def test_pt_production_large_instances(gpu, kind, kw):
     assert_same(g, o, exact_energy=True)
 
 
+@pytest.mark.parametrize("kind,kw,B,C,T", [("2d_pm", dict(L=182), 32, 2, 6), ("rr3", dict(n=70000), 32, 2, 4),
+                                           ("3d_gauss", dict(L=20), 96, 2, 4), ("2d_pm", dict(L=100), 96, 3, 4)])
+def test_pt_production_spins_in_global_memory(gpu, kind, kw, B, C, T):
+    # chain pairs whose spin planes do not fit in LDS: 33,124 sites at 64 replicas (265 KB of
+    # planes), 70,000 sites (beyond the 16-bit neighbour lists, so ICM walks the coupling CSR), and
+    # 8,000 and 10,000 sites at 192 replicas (three plane words per site; with three chains the
+    # third runs alone).  The planes stay in the group's global state array and the clusters grow
+    # in global scratch.
+    g, o = _run_both(kind, kw, B=B, C=C, T=T, lo=B // 2, sample=2, sstates=3, limiting=2)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=True)
+
+
 def test_icm_growth_every_sites_per_thread(gpu):
     # one wave per work group (64 threads): the cluster growth keeps 4, 8 or 16 sites per thread in
     # registers, for neighbour rows of one 64-bit word (width 4) and of two (3D lattices, width 6);
