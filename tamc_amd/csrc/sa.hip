@@ -21,6 +21,7 @@
 #include <chrono>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "runtime_common.hpp"
@@ -71,16 +72,22 @@ __host__ __device__ __forceinline__ uint32_t sa_plane_words(uint32_t n) { return
 __host__ __device__ __forceinline__ uint32_t sa_wave_words(uint32_t n) { return sa_plane_words(n) + kRowBufWords; }
 
 // One wave: 64 restarts of one solve, from their initial states to their final energies.
-// sp: this wave's LDS, [n][2] plane words followed by the row buffer [kRowBufWords].
-__device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) {
+// lds: this wave's LDS, [n][2] plane words followed by the row buffer [kRowBufWords]; with kG the
+// plane words stay in the task's global spins_out (an instance too large for LDS) and the LDS holds
+// the row buffer only.
+template <bool kG>
+__device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *lds) {
+  using PW = std::conditional_t<kG, uint32_t, TAMC_AS3 uint32_t>;
+  using PC = std::conditional_t<kG, const char, const TAMC_AS3 char>;
   const uint32_t lane = sa_lane(), half = lane >> 5, sh = lane & 31u, up = 31u - sh;
   const uint32_t n = X.n, nq = (n + 3u) / 4u;
   const uint32_t r = X.rbase + lane;
   const bool valid = r < X.R;
   const uint32_t key0 = (uint32_t)X.seed, key1 = (uint32_t)(X.seed >> 32);
+  PW *sp = kG ? (PW *)X.spins_out : (PW *)lds;
   // this lane's half of every plane word: byte offsets are 8 * site + 4 * half
-  const TAMC_AS3 char *spl = (const TAMC_AS3 char *)sp + half * 4u;
-  TAMC_AS3 W4 *rb = (TAMC_AS3 W4 *)(sp + sa_plane_words(n));
+  PC *spl = (PC *)sp + half * 4u;
+  TAMC_AS3 W4 *rb = (TAMC_AS3 W4 *)(lds + (kG ? 0u : sa_plane_words(n)));
   const W4 *ginfo = (const W4 *)X.info;
   const W4 *gblk = (const W4 *)X.blocks;  // block b = W4 4b .. 4b+3
   auto put = [&](uint32_t i, uint64_t m) {
@@ -89,7 +96,8 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
   if (X.k0 > 0u) {
     // continue from the plane words the previous launch left (a wave's LDS writes and reads stay
     // in order, so no barrier is needed)
-    for (uint32_t w = lane; w < n * 2u; w += 64u) sp[w] = X.spins_out[w];
+    if (!kG)
+      for (uint32_t w = lane; w < n * 2u; w += 64u) sp[w] = X.spins_out[w];
   } else {
   // initial states: rand_ising_state with the production init stream (one Philox block per quad)
   for (uint32_t q = 0; q < nq; ++q) {
@@ -144,7 +152,7 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
               }
             }
 #pragma unroll
-            for (int e = 0; e < 32; ++e) w[e] = *(const TAMC_AS3 uint32_t *)(spl + off[e]);
+            for (int e = 0; e < 32; ++e) w[e] = *(const PW *)(spl + off[e]);
             // d += J * s_j in row order; J * (+-1.0) is exact, so each fma rounds like the add
 #pragma unroll
             for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jb[e]), sa_sign(w[e], up), d);
@@ -154,8 +162,8 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
             uint32_t w[8];
 #pragma unroll
             for (uint32_t e = 0; e < 4u; ++e) {
-              w[e] = *(const TAMC_AS3 uint32_t *)(spl + o0[e]);
-              w[4u + e] = *(const TAMC_AS3 uint32_t *)(spl + o1[e]);
+              w[e] = *(const PW *)(spl + o0[e]);
+              w[4u + e] = *(const PW *)(spl + o1[e]);
             }
 #pragma unroll
             for (uint32_t e = 0; e < 4u; ++e) d = __fmaf_rn(__uint_as_float(j0[e]), sa_sign(w[e], up), d);
@@ -176,7 +184,8 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
     }
   }
   if (X.k1 < X.K) {  // more sweeps follow in a later launch: keep the planes
-    for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
+    if (!kG)
+      for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
     return;
   }
   // AnnealMinResults::apply_measurements: energy_ref of every final state (src/sa.rs:55-64), the
@@ -187,13 +196,13 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
   for (uint32_t i = 0; i < n; ++i) {
     const TAMC_AS4 uint32_t *inf = info + 4u * i;
     const uint32_t b0 = inf[0], len = inf[3];
-    const float si = sa_sign(*(const TAMC_AS3 uint32_t *)(spl + 8u * i), up);
+    const float si = sa_sign(*(const PW *)(spl + 8u * i), up);
     float h = 0.0f;
     h += __uint_as_float(inf[2]);
     const TAMC_AS4 uint32_t *blk = blocks + (size_t)b0 * 16u;
     for (uint32_t e = 0; e < len; ++e) {
       const uint32_t bl = e >> 3, k = e & 7u;
-      const float sj = sa_sign(*(const TAMC_AS3 uint32_t *)(spl + blk[bl * 16u + k]), up);
+      const float sj = sa_sign(*(const PW *)(spl + blk[bl * 16u + k]), up);
       h += __fmul_rn(__uint_as_float(blk[bl * 16u + 8u + k]), sj) / 2.0f;
     }
     total = __fadd_rn(total, __fmul_rn(h, si));
@@ -207,7 +216,8 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *sp) 
     return;
   }
   X.energy_out[lane] = total;
-  for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
+  if (!kG)
+    for (uint32_t w = lane; w < n * 2u; w += 64u) X.spins_out[w] = sp[w];
 }
 
 // Work groups are spread over the XCDs round robin (block b runs on XCD b mod 8); the logical
@@ -218,6 +228,9 @@ __device__ __forceinline__ uint32_t sa_logical_group() {
   return (blockIdx.x % 8u) * per + blockIdx.x / 8u;
 }
 
+// kG: the instances' plane words stay in global memory (some instance of the launch is too large
+// for LDS), the LDS holds the row buffers only
+template <bool kG>
 __global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t ntasks, uint32_t words_per_wave,
                                                  uint32_t ngroups) {
   extern __shared__ __align__(16) uint32_t sa_smem[];
@@ -226,7 +239,7 @@ __global__ __launch_bounds__(256) void sa_kernel(const SaTask *tasks, uint32_t n
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64u);
   const uint32_t tix = g * (blockDim.x / 64u) + wave;
   if (tix >= ntasks) return;
-  sa_wave(tasks[tix], (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
+  sa_wave<kG>(tasks[tix], (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
 }
 
 // One solve whose waves are described by one task passed by value: wave w takes restarts
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(256) void sa_kernel_direct(SaTask base, uint32_t nt
   X.rbase = tix * 64u;
   X.energy_out = base.energy_out + X.rbase;
   X.states_out = base.states_out + (size_t)X.rbase * base.n;
-  sa_wave(X, (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
+  sa_wave<false>(X, (TAMC_AS3 uint32_t *)sa_smem + (size_t)wave * words_per_wave);
 }
 
 // Scratch of the allocation-free entry point: the schedule's betas are staged here.  Host code
@@ -259,7 +272,6 @@ tamc_status sa_validate(const tamc_instance *inst, const tamc_sa_params &P) {
   if (P.num_replicas > 1 && P.threads == 0) return fail(TAMC_E_INVALID_ARG, "threads must be >= 1");
   for (uint32_t k = 1; k < P.num_betas; ++k)
     if (!(P.betas[k] - P.betas[k - 1] >= 0.0f)) return fail(TAMC_E_INVALID_ARG, "beta array must be non-decreasing");
-  if (!sa_wave_lds_words(inst->n)) return fail(TAMC_E_UNSUPPORTED, "instance too large for LDS-resident SA (n > 20352)");
   return TAMC_OK;
 }
 
@@ -281,12 +293,24 @@ static tamc_status sa_lds_attribute(K kernel, uint32_t lds) {
   return TAMC_OK;
 }
 
+// LDS words per wave of a launch whose instances have at most maxn sites: the planes and the row
+// buffer, or the row buffer alone when the planes stay in global memory
+static uint32_t sa_launch_wave_words(uint32_t maxn) {
+  return sa_wave_lds_words(maxn) ? sa_wave_words(maxn) : kRowBufWords;
+}
+
 tamc_status sa_launch(const SaTask *d_tasks, uint32_t ntasks, uint32_t maxn, hipStream_t s) {
   if (!ntasks) return TAMC_OK;
-  const uint32_t wpw = sa_wave_words(maxn), wpg = sa_waves_per_group(wpw), lds = wpg * wpw * 4u;
-  if (tamc_status st = sa_lds_attribute(sa_kernel, lds)) return st;
+  const bool global = sa_wave_lds_words(maxn) == 0u;
+  const uint32_t wpw = sa_launch_wave_words(maxn), wpg = sa_waves_per_group(wpw), lds = wpg * wpw * 4u;
   const uint32_t ngroups = (ntasks + wpg - 1) / wpg;
-  hipLaunchKernelGGL(sa_kernel, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, d_tasks, ntasks, wpw, ngroups);
+  if (global) {
+    if (tamc_status st = sa_lds_attribute(sa_kernel<true>, lds)) return st;
+    hipLaunchKernelGGL(sa_kernel<true>, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, d_tasks, ntasks, wpw, ngroups);
+  } else {
+    if (tamc_status st = sa_lds_attribute(sa_kernel<false>, lds)) return st;
+    hipLaunchKernelGGL(sa_kernel<false>, dim3(sa_grid(ngroups)), dim3(64 * wpg), lds, s, d_tasks, ntasks, wpw, ngroups);
+  }
   TAMC_HIP(hipGetLastError());
   return TAMC_OK;
 }
@@ -397,7 +421,7 @@ tamc_status run_sa(std::vector<SaSolve> &solves, tamc_sa_result **results, tamc_
     stats->device_ms = ms;
     stats->wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
     stats->attempts = attempts;
-    stats->waves_per_group = sa_waves_per_group(sa_wave_words(maxn));
+    stats->waves_per_group = sa_waves_per_group(sa_launch_wave_words(maxn));
     stats->launches = 1;
     stats->groups = ntasks;
   }
@@ -414,6 +438,8 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   if (!inst || !params || !params->betas || !energies || !states) return fail(TAMC_E_INVALID_ARG, "null argument");
   const tamc_sa_params &P = *params;
   if (tamc_status st = sa_validate(inst, P)) return st;
+  if (!sa_wave_lds_words(inst->n))
+    return fail(TAMC_E_UNSUPPORTED, "the allocation-free SA keeps its planes in LDS (n <= 20352)");
   if ((size_t)P.num_betas * 4u > kNoAllocScratchBytes)
     return fail(TAMC_E_INVALID_ARG, "schedule longer than the allocation-free scratch holds (131072 betas)");
   const uint32_t n = inst->n;

@@ -291,6 +291,20 @@ def test_sa_bit_exact(gpu, kind, kw, R, threads):
     assert g["min_energy"] == o["min_energy"]
 
 
+@pytest.mark.parametrize("kind,kw,R", [("rr3", dict(n=30000), 96), ("qubo", dict(n=21000, density=0.0004), 64)])
+def test_sa_planes_in_global_memory(gpu, kind, kw, R):
+    # instances whose 64 restarts' bit planes (8 bytes per site) do not fit in LDS: the planes stay
+    # in the wave's global plane buffer, the row buffer in LDS
+    oi, gi, _ = make_pair(kind, seed=4, **kw)
+    betas = np.linspace(0.1, 4.0, 6).astype(np.float32)
+    params = api.SaParams(beta=api.BetaOptions.Arr(betas), num_replicas=R, threads=1, seed=78)
+    g = api.SaRunner(gi, params).run()
+    o = O.sa_run(oi, betas, R, threads=1, rng_mode=O.RNG_PHILOX, seed=78)
+    assert np.array_equal(g["final_states"], o["final_states"])
+    assert np.array_equal(g["energies"], o["energies"])
+    assert g["min_energy"] == o["min_energy"]
+
+
 def test_batch_equals_single_and_plain(gpu):
     insts, params = [], []
     for k, n in enumerate([40, 100, 64, 200]):

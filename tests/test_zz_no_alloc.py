@@ -69,3 +69,7 @@ def test_noalloc_sa_rejects_what_it_cannot_hold(gpu):
     with pytest.raises(_lib.TamcError) as e:
         _noalloc(fresh, api.SaParams(beta=api.BetaOptions.Arr([0.5]), num_replicas=64, seed=1), 64)
     assert e.value.status == _lib.TAMC_E_INVALID_ARG
+    _, big, _ = make_pair("rr3", seed=3, n=21000)  # its planes would need global memory: refused
+    with pytest.raises(_lib.TamcError) as e:
+        _noalloc(big, api.SaParams(beta=api.BetaOptions.Arr([0.5]), num_replicas=64, seed=1), 64)
+    assert e.value.status == _lib.TAMC_E_UNSUPPORTED
