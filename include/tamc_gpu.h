@@ -418,7 +418,8 @@ tamc_status tamc_pt_opt_update(uint32_t nt, const float *beta_vec, float *moment
  * Philox4x32-10 keyed by seed, counter {k, i, 0, 4}.  max_cluster[k] = size of the largest
  * connected cluster of occupied sites (0 when none is occupied, where the reference panics);
  * percolates[k] = 1 when an occupied site of negative weight is connected to an occupied site of
- * positive weight (the reference's depth-first search finds a path).  At most 16384 sites. */
+ * positive weight (the reference's depth-first search finds a path).  Any number of sites (above
+ * 16384 the labels live in device scratch, samples run in batches). */
 tamc_status tamc_percolate(const tamc_instance *inst, double p, uint32_t num_samples, uint64_t seed,
                            uint32_t *max_cluster, uint8_t *percolates);
 

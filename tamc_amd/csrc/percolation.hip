@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -18,7 +19,9 @@ using namespace tamc;
 namespace {
 
 constexpr uint32_t kFree = 0xFFFFFFFFu;      // unoccupied site
-constexpr uint32_t kPercMaxSites = 16384u;   // two 32-bit words per site in LDS
+constexpr uint32_t kPercLdsSites = 16384u;   // two 32-bit words per site in LDS up to this size; larger
+                                             // instances keep them in global scratch
+constexpr size_t kPercScratchBytes = 256u << 20;  // global scratch of one launch (samples in batches)
 
 struct PercArgs {
   const uint32_t *row_ptr, *col;  // coupling pattern (no self entries in Ising form)
@@ -38,10 +41,13 @@ __device__ __forceinline__ bool occupied(const PercArgs &A, uint32_t k, uint32_t
   return d < A.p_int;
 }
 
-__global__ __launch_bounds__(256) void percolation_kernel(PercArgs A) {
+// Work group b handles sample k0 + b.  kG: its labels and counters are the slice [2n) of the global
+// scratch at 2nb (an instance too large for LDS); otherwise they are in LDS.
+template <bool kG>
+__global__ __launch_bounds__(256) void percolation_kernel(PercArgs A, uint32_t k0, uint32_t *scratch) {
   extern __shared__ __align__(16) uint32_t lds[];
-  uint32_t *label = lds, *aux = lds + A.n;
-  const uint32_t k = blockIdx.x, n = A.n;
+  const uint32_t k = k0 + blockIdx.x, n = A.n;
+  uint32_t *label = kG ? scratch + 2u * (size_t)n * blockIdx.x : lds, *aux = label + n;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     label[i] = occupied(A, k, i) ? i : kFree;
     aux[i] = 0u;
@@ -105,8 +111,6 @@ extern "C" tamc_status tamc_percolate(const tamc_instance *inst, double p, uint3
   if (!inst || (num_samples && (!max_cluster || !percolates))) return fail(TAMC_E_INVALID_ARG, "null argument");
   if (!(p >= 0.0 && p <= 1.0)) return fail(TAMC_E_INVALID_ARG, "Bernoulli probability must be in [0, 1]");
   if (inst->qubo) return fail(TAMC_E_INVALID_ARG, "percolation takes an instance read in Ising form");
-  if (inst->n > kPercMaxSites)
-    return fail(TAMC_E_UNSUPPORTED, "percolation supports at most " + std::to_string(kPercMaxSites) + " sites");
   tamc_status st = device_init();
   if (st) return st;
   if (num_samples == 0 || inst->n == 0) {
@@ -132,12 +136,25 @@ extern "C" tamc_status tamc_percolate(const tamc_instance *inst, double p, uint3
   A.p_int = (p >= 1.0) ? ~0ull : (uint64_t)(p * 18446744073709551616.0);
   A.max_cluster = (uint32_t *)out_c.p;
   A.percolates = (uint8_t *)out_p.p;
-  const size_t lds = (size_t)inst->n * 8u;
-  if (lds > 65536u)
-    TAMC_HIP(hipFuncSetAttribute((const void *)percolation_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
-  hipLaunchKernelGGL(percolation_kernel, dim3(num_samples), dim3(256), lds, 0, A);
-  TAMC_HIP(hipGetLastError());
+  if (inst->n <= kPercLdsSites) {
+    const size_t lds = (size_t)inst->n * 8u;
+    if (lds > 65536u)
+      TAMC_HIP(hipFuncSetAttribute((const void *)percolation_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    hipLaunchKernelGGL(percolation_kernel<false>, dim3(num_samples), dim3(256), lds, 0, A, 0u, (uint32_t *)nullptr);
+    TAMC_HIP(hipGetLastError());
+  } else {
+    // labels and counters in global scratch, as many samples per launch as it holds
+    const size_t per = (size_t)inst->n * 8u;
+    const uint32_t batch = (uint32_t)std::max<size_t>(1, std::min<size_t>(num_samples, kPercScratchBytes / per));
+    DevBuf scr;
+    TAMC_HIP(scr.alloc(per * batch));
+    for (uint32_t k0 = 0; k0 < num_samples; k0 += batch) {
+      const uint32_t nb = std::min(batch, num_samples - k0);
+      hipLaunchKernelGGL(percolation_kernel<true>, dim3(nb), dim3(256), 0, 0, A, k0, (uint32_t *)scr.p);
+      TAMC_HIP(hipGetLastError());
+    }
+  }
   TAMC_HIP(hipMemcpy(max_cluster, out_c.p, (size_t)num_samples * 4, hipMemcpyDeviceToHost));
   TAMC_HIP(hipMemcpy(percolates, out_p.p, num_samples, hipMemcpyDeviceToHost));
   return TAMC_OK;

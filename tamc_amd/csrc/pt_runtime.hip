@@ -66,7 +66,7 @@ tamc_status plan_solve(SolvePlan &S, const tamc_instance *inst, const tamc_pt_pa
     g.nch = std::min(2u, S.C - c0);
     g.R = g.nch * S.B;
     if (g.R > kMaxGroupReplicas)
-      return fail(TAMC_E_UNSUPPORTED, "a chain pair may hold at most 256 replicas (num_betas <= 128)");
+      return fail(TAMC_E_UNSUPPORTED, "a chain pair may hold at most 1024 replicas (num_betas <= 512)");
     g.G = (g.R + 63) / 64;
     g.P = (P->icm && g.nch == 2 && P->lo_beta_idx < S.B) ? S.B - P->lo_beta_idx : 0;
     // the group's state in LDS when it fits; else the ICM scratch, then also the spin planes, in

@@ -11,7 +11,7 @@
 namespace tamc {
 
 constexpr uint32_t kMaxWaves = 16;       // waves per work group (1024 threads)
-constexpr uint32_t kMaxGroupReplicas = 256;
+constexpr uint32_t kMaxGroupReplicas = 1024;  // 16 plane words per site
 constexpr uint32_t kMaxPairs = 32;       // ICM pairs per group held in one 32-bit cluster word
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 

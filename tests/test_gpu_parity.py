@@ -172,6 +172,16 @@ def test_pt_production_spins_in_global_memory(gpu, kind, kw, B, C, T):
     assert_same(g, o, exact_energy=True)
 
 
+@pytest.mark.parametrize("kind,kw,B,C,threads", [("2d_pm", dict(L=8), 200, 2, 1), ("rr3", dict(n=100), 300, 3, 4),
+                                                   ("2d_gauss", dict(L=6), 512, 2, 1)])
+def test_pt_production_long_ladders(gpu, kind, kw, B, C, threads):
+    # ladders of 200 to 512 betas: chain pairs of 400 to 1024 replicas (7 to 16 plane words per
+    # site), exchange walked by one lane, ICM over several 32-pair chunks
+    g, o = _run_both(kind, kw, B=B, C=C, T=12, lo=B // 3, threads=threads, sample=3, sstates=4, limiting=1)
+    assert o["error"] == 0
+    assert_same(g, o, exact_energy=True)
+
+
 def test_icm_growth_every_sites_per_thread(gpu):
     # one wave per work group (64 threads): the cluster growth keeps 4, 8 or 16 sites per thread in
     # registers, for neighbour rows of one 64-bit word (width 4) and of two (3D lattices, width 6);

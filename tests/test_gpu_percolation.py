@@ -38,6 +38,22 @@ def test_percolation_matches_restatement(gpu, case):
         assert 0.2 < pc.mean() < 0.9
 
 
+def test_percolation_beyond_lds(gpu):
+    # 30,000 sites: labels and counters in global scratch instead of LDS
+    rng = np.random.default_rng(12)
+    i, j, K, n = W.random_regular3(30000, rng, gaussian=False, bias_sigma=0.0)
+    w = rng.integers(-1, 2, n).astype(np.float64)
+    i, j, K = np.concatenate([i, np.arange(n)]), np.concatenate([j, np.arange(n)]), np.concatenate([K, w])
+    p, samples, seed = 0.5, 6, 31
+    inst = api.BqmIsingInstance.from_lines(i, j, K)
+    oi = O.Instance.from_lines(i, j, K)
+    cl, pc = api.random_percolation(inst, p, samples, seed)
+    for k in range(samples):
+        occ = OP.occupancy(n, p, k, seed)
+        perc, gmax = OP.random_percolation(oi.row_ptr, oi.col, oi.bias, occ)
+        assert (int(cl[k]), bool(pc[k])) == (gmax, perc), k
+
+
 def test_percolation_edge_probabilities(gpu):
     i, j, K, n = _square(8)
     inst = api.BqmIsingInstance.from_lines(i, j, K)
