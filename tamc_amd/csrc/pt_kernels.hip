@@ -341,6 +341,31 @@ __device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
   return v;
 }
 
+// Minimum over the 64 lanes, in every lane (every lane of the wave must be active): the same DPP
+// pattern as wave_incl_scan with +inf (or all ones) for lanes that have no source, then lane 63.
+__device__ __forceinline__ float wave_min_f(float x) {
+  const int inf = 0x7F800000;
+  int v = __float_as_int(x);
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x111, 0xf, 0xf, false))));
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x112, 0xf, 0xf, false))));
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x114, 0xf, 0xf, false))));
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x118, 0xf, 0xf, false))));
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x142, 0xa, 0xf, false))));
+  v = __float_as_int(fminf(__int_as_float(v), __int_as_float(__builtin_amdgcn_update_dpp(inf, v, 0x143, 0xc, 0xf, false))));
+  return __int_as_float(__builtin_amdgcn_readlane(v, 63));
+}
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t x) {
+  const int none = -1;
+  int v = (int)x;
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x111, 0xf, 0xf, false));
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x112, 0xf, 0xf, false));
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x114, 0xf, 0xf, false));
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x118, 0xf, 0xf, false));
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x142, 0xa, 0xf, false));
+  v = (int)min((uint32_t)v, (uint32_t)__builtin_amdgcn_update_dpp(none, v, 0x143, 0xc, 0xf, false));
+  return (uint32_t)__builtin_amdgcn_readlane(v, 63);
+}
+
 __device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t src) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src << 2), (int)v);
 }
@@ -1621,32 +1646,36 @@ __device__ __forceinline__ void measure_phase(Ctx &c, uint32_t t) {
       }
     }
   }
-  // ground state: first minimum over chains then slots of this group; record strict improvements
-  if (c.tid == 0) {
-    float best = 0.f;
-    uint32_t br = 0;
-    bool first = true;
-    for (uint32_t ch = 0; ch < c.nch; ++ch)
-      for (uint32_t b = 0; b < B; ++b) {
-        const uint32_t r = c.perm[ch * B + b];
-        const float e = c.energy[r];
-        if (first || e < best) {
-          best = e;
-          br = r;
-          first = false;
-        }
+  // ground state: the first minimum over chains, then slots, of this group's energies (slot index
+  // k = chain * B + slot); strict improvements are recorded.  Wave 0: each lane takes the first
+  // minimum of its slots k = lane (mod 64), then the wave the smallest value and, among the lanes
+  // holding it, the smallest index.  The record count and minimum live in misc[2], misc[3].
+  if (c.wave == 0u) {
+    const uint32_t NB = c.nch * B;
+    float best = __uint_as_float(0x7F800000u);  // +inf
+    uint32_t bk = kNone;
+    for (uint32_t k = c.lane; k < NB; k += 64u) {
+      const float e = c.energy[c.perm[k]];
+      if (bk == kNone || e < best) {
+        best = e;
+        bk = k;
       }
-    const uint32_t cnt = X.gs_count[0];
-    uint32_t rec = kNone;
-    if ((cnt == 0u || best < X.gs_min[0]) && cnt < X.gs_cap) {
-      rec = cnt;
-      X.gs_count[0] = cnt + 1u;
-      X.gs_min[0] = best;
-      X.out_gs_steps[cnt] = t;
-      X.out_gs_e[cnt] = best;
     }
-    c.misc[0] = rec;
-    c.misc[1] = br;
+    const float m = wave_min_f(best);
+    const uint32_t k = wave_min_u((bk != kNone && best == m) ? bk : kNone);
+    if (c.lane == 0u) {
+      const uint32_t cnt = c.misc[2];
+      uint32_t rec = kNone;
+      if ((cnt == 0u || m < __uint_as_float(c.misc[3])) && cnt < X.gs_cap) {
+        rec = cnt;
+        c.misc[2] = cnt + 1u;
+        c.misc[3] = __float_as_uint(m);
+        X.out_gs_steps[cnt] = t;
+        X.out_gs_e[cnt] = m;
+      }
+      c.misc[0] = rec;
+      c.misc[1] = c.perm[k];
+    }
   }
   __syncthreads();
   const uint32_t rec = c.misc[0];
@@ -1763,6 +1792,8 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
     c.moves[0] = 0u;
     for (int k = 0; k <= (int)kPhaseSlots; ++k) c.ticks[k] = 0ull;
     c.misc[15] = c.members > 1u ? X.team_ctr[3u + c.member] : 0u;  // team meetings so far
+    c.misc[2] = X.gs_count[0];  // the ground-state tracker, kept in LDS during the launch
+    c.misc[3] = __float_as_uint(X.gs_min[0]);
   }
   // load persistent state
   const uint32_t nw = c.n * c.G * 2u;
@@ -1818,7 +1849,11 @@ __device__ __forceinline__ void run_group(const PtGroup *group, uint32_t t0, uin
   if (c.tid == 0 && X.phase_ticks)
     for (int k = 0; k < (int)kPhaseSlots; ++k) X.phase_ticks[k] += c.ticks[k];
   __syncthreads();
-  if (c.tid == 0) X.icm_moves[0] += c.moves[0];
+  if (c.tid == 0) {
+    X.icm_moves[0] += c.moves[0];
+    X.gs_count[0] = c.misc[2];
+    X.gs_min[0] = __uint_as_float(c.misc[3]);
+  }
   if (c.tid == 0 && launch_ticks) launch_ticks[blockIdx.x] = wall_clock64() - launch_start;
   // store persistent state
   if (spins_lds)

@@ -168,8 +168,8 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(R * 4u);
   L.seed_blk = o;
   o += lds_align(kMaxPairs * 4u);
-  L.misc = o;  // [0..1] measurement, [4] ICM moves, [8..10] growth pass flags, [12..14] round flags
-                // of block_any, [15] team meetings
+  L.misc = o;  // [0..1] measurement, [2..3] ground-state record count and minimum, [4] ICM moves,
+                // [8..10] growth pass flags, [12..14] round flags of block_any, [15] team meetings
   o += lds_align(16u * 4u);
   L.ticks = o;  // phase timers [kPhaseSlots] + last timestamp
   o += lds_align((kPhaseSlots + 1u) * 8u);
