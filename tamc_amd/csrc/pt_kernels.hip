@@ -1265,17 +1265,25 @@ __device__ __forceinline__ void exchange_phase(Ctx &c, uint32_t t) {
       const bool ok = step && (dlt >= 0.0f || accept_u(dlt, w));
       okmask = ballot(ok);
     } else {
+      // every step decided at once as if the step before it rejected (its dE unchanged); the walk
+      // then decides again only the steps that follow an accepted swap
+      const float dlt0 = dbeta * dE;
+      const uint64_t spec = ballot(step && (dlt0 >= 0.0f || accept_u(dlt0, w)));
       okmask = 0ull;
       bool prev_ok = false;
       float prev = 0.0f;
       for (uint32_t k = 0; k < B1; ++k) {
         const float dEk = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(dE), k));
-        const float dbk = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(dbeta), k));
-        const uint32_t wk = __builtin_amdgcn_readlane(w, k);
-        // after a swap at k-1, dE[k] += dE[k-1] (the value used at k-1)
-        const float eff = prev_ok ? dEk + prev : dEk;
-        const float dlt = dbk * eff;
-        const bool ok = dlt >= 0.0f || accept_u(dlt, wk);
+        float eff = dEk;
+        bool ok = ((spec >> k) & 1ull) != 0ull;
+        if (prev_ok) {
+          // after a swap at k-1, dE[k] += dE[k-1] (the value used at k-1)
+          const float dbk = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(dbeta), k));
+          const uint32_t wk = __builtin_amdgcn_readlane(w, k);
+          eff = dEk + prev;
+          const float dlt = dbk * eff;
+          ok = dlt >= 0.0f || accept_u(dlt, wk);
+        }
         if (ok) okmask |= 1ull << k;
         prev_ok = ok;
         prev = eff;
