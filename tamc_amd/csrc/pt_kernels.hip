@@ -160,7 +160,7 @@ struct Ctx {
   uint32_t *sp;      // LDS bit planes
   float *energy, *beta_of, *dEx;
   long long *partial, *drift;
-  uint32_t *perm, *pair_a, *pair_b, *seed_blk, *misc;
+  uint32_t *perm, *pair_a, *pair_b, *pair_of, *seed_blk, *misc;
   uint32_t *moves;   // LDS count of ICM moves in this launch
   uint8_t *tags;
   uint32_t *nacc, *hist, *rt;
@@ -274,25 +274,6 @@ __device__ __forceinline__ long long tag_sum(const unsigned long long *sums, siz
   }
   return s;
 }
-
-// Spin bits of one site for replica indices chosen at run time.  With one 64-bit plane word per
-// site (at most 64 replicas) the word is held in a register and a bit is one 64-bit shift; wider
-// groups read the bit from LDS.  (Selecting among several registers by a run-time index makes the
-// compiler spill them to scratch memory.)
-struct SiteWords {
-  uint64_t w;
-  const uint32_t *sp;
-  uint32_t i, G;
-  __device__ __forceinline__ void load(const uint32_t *sp_, uint32_t i_, uint32_t G_) {
-    sp = sp_;
-    i = i_;
-    G = G_;
-    w = (G == 1u) ? (((uint64_t)sp[i * 2u + 1u] << 32) | sp[i * 2u]) : 0ull;
-  }
-  __device__ __forceinline__ uint32_t bit(uint32_t r) const {
-    return (G == 1u) ? (uint32_t)((w >> r) & 1ull) : sbit(sp, G, i, r);
-  }
-};
 
 // Inclusive prefix sum over the 64 lanes with DPP row shifts and row broadcasts.  Every lane of
 // the wave must be active.
@@ -1030,31 +1011,53 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     c.fmask[idx] = m;  // read only after the growth loop's barriers
   }
   // (a) overlap words: bit p of qw[i] <=> the replicas of pair p disagree at site i; blk[p][b] =
-  //     number of such sites in 64-site block b
+  //     number of such sites in 64-site block b.  Bit p of q is bit a_p ^ bit b_p of the site's
+  //     planes.  With one plane word per site the word is held in a register and shifted per pair.
+  //     With two to kMaxTableWords words, q is the XOR, over the bytes of the plane words, of a
+  //     table entry per byte: tab[g][j][v] = XOR of pm(r) over the bits k of v, r = 64 g + 8 j + k,
+  //     where pm(r) = 1 << p when r is replica a_p or b_p of a pair of the chunk (else 0); the
+  //     tables (8 KB per plane word) live in the partial-sum area, which only the sweep and the
+  //     measurements use.  With more words each pair's two bits are read from the planes.
+  const bool tables = G > 1u && G <= kMaxTableWords;
+  uint32_t *tab = reinterpret_cast<uint32_t *>(c.partial);
+  if (tables) {
+    uint32_t *pm = c.pair_of;
+    for (uint32_t r = c.tid; r < c.R; r += c.nthreads) pm[r] = 0u;
+    __syncthreads();
+    if (c.tid < P) {
+      pm[c.pair_a[c.tid]] = 1u << c.tid;
+      pm[c.pair_b[c.tid]] = 1u << c.tid;
+    }
+    __syncthreads();
+    for (uint32_t idx = c.tid; idx < G * 2048u; idx += c.nthreads) {
+      const uint32_t r0 = idx >> 5 & ~7u, v = idx & 255u;  // r0 = 64 g + 8 j
+      uint32_t t = 0u;
+#pragma unroll
+      for (uint32_t k = 0; k < 8u; ++k)
+        if (((v >> k) & 1u) && r0 + k < c.R) t ^= pm[r0 + k];
+      tab[idx] = t;
+    }
+    __syncthreads();
+  }
   for (uint32_t blk = c.wave; blk < c.nblk; blk += c.W) {
     const uint32_t i = blk * 64u + c.lane;
     const bool valid = i < n;
     uint32_t q = 0;
     if (valid && G == 1u) {
-      // one 64-bit plane word per site: bit p = bit a_p ^ bit b_p of the word
       const uint64_t w = *reinterpret_cast<const uint64_t *>(c.sp + 2u * i);
       for (uint32_t p = 0; p < P; ++p) q |= (uint32_t)(((w >> c.pair_a[p]) ^ (w >> c.pair_b[p])) & 1ull) << p;
-      M.qw[i] = q;
-      M.cw[i] = 0u;
-    } else if (valid) {
-      SiteWords h{0ull, c.sp, 0u, G};
-      h.load(c.sp, i, G);
-      for (uint32_t p0 = 0; p0 < P; p0 += 4u) {
-        const uint4 A = reinterpret_cast<const uint4 *>(c.pair_a)[p0 >> 2];
-        const uint4 Bp = reinterpret_cast<const uint4 *>(c.pair_b)[p0 >> 2];
-        const uint32_t av[4] = {A.x, A.y, A.z, A.w}, bv[4] = {Bp.x, Bp.y, Bp.z, Bp.w};
-#pragma unroll
-        for (uint32_t s = 0; s < 4u; ++s) {
-          const uint32_t p = p0 + s;
-          if (p >= P) break;
-          q |= (h.bit(av[s]) ^ h.bit(bv[s])) << p;
-        }
+    } else if (valid && tables) {
+      for (uint32_t g = 0; g < G; ++g) {
+        const uint32_t lo = c.sp[(i * G + g) * 2u], hi = c.sp[(i * G + g) * 2u + 1u];
+        const uint32_t *tg = tab + g * 2048u;
+        q ^= tg[lo & 255u] ^ tg[256u + ((lo >> 8) & 255u)] ^ tg[512u + ((lo >> 16) & 255u)] ^ tg[768u + (lo >> 24)];
+        q ^= tg[1024u + (hi & 255u)] ^ tg[1280u + ((hi >> 8) & 255u)] ^ tg[1536u + ((hi >> 16) & 255u)] ^
+             tg[1792u + (hi >> 24)];
       }
+    } else if (valid) {
+      for (uint32_t p = 0; p < P; ++p) q |= (sbit(c.sp, G, i, c.pair_a[p]) ^ sbit(c.sp, G, i, c.pair_b[p])) << p;
+    }
+    if (valid) {
       M.qw[i] = q;
       M.cw[i] = 0u;
     }
@@ -1749,6 +1752,7 @@ __device__ __forceinline__ Ctx make_ctx(const PtGroup *group, uint32_t icm_lds, 
   c.rt = (uint32_t *)(smem + L.rt);
   c.pair_a = (uint32_t *)(smem + L.pair_a);
   c.pair_b = (uint32_t *)(smem + L.pair_b);
+  c.pair_of = (uint32_t *)(smem + L.pair_of);
   c.seed_blk = (uint32_t *)(smem + L.seed_blk);
   c.misc = (uint32_t *)(smem + L.misc);
   c.partial = (long long *)(smem + L.partial);

@@ -14,6 +14,7 @@ constexpr uint32_t kMaxWaves = 16;       // waves per work group (1024 threads)
 constexpr uint32_t kMaxGroupReplicas = 1024;  // 16 plane words per site
 constexpr uint32_t kMaxPairs = 32;       // ICM pairs per group held in one 32-bit cluster word
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kMaxTableWords = 4;   // ICM overlap by byte tables for 2 to 4 plane words per site
 
 // Per-group phase timers (100 MHz wall-clock ticks of the group's thread 0); the last slot counts
 // ICM cluster-growth passes.
@@ -177,8 +178,10 @@ __host__ __device__ inline LdsLayout make_layout(uint32_t n, uint32_t G, uint32_
   o += lds_align(P > 0 ? G * (kMaxPairs / 4u * 16u * 8u) : 8u);
   L.fscr = o;  // ICM cluster labelling: a site of each straggling pair's partial cluster
   o += lds_align(P > 0 ? kMaxPairs * 4u : 4u);
-  L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G)
-  o += lds_align(W * G * 64u * 8u);
+  L.partial = o;  // per-wave partial sums: W*G*64 int64 (also W*B u32 counters since B <= 64*G);
+                  // during ICM the overlap tables, 8 KB per plane word (two to four words)
+  const uint32_t tabs = (P > 0 && G > 1u && G <= kMaxTableWords) ? G * 8192u : 0u;
+  o += lds_align(W * G * 64u * 8u > tabs ? W * G * 64u * 8u : tabs);
   L.dE = o;  // exchange energy differences [2][B]
   o += lds_align(2u * B * 4u);
   L.drift = o;
