@@ -567,8 +567,11 @@ __device__ __forceinline__ void min_half(U32P w, uint32_t h, uint32_t l) {
   }
 }
 
+// spt: sites per thread (site tid + s * nthreads for s < spt); S only sets how many sites' rows are
+// requested together.
 template <uint32_t S, uint32_t RW, bool kLds>
-__device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t pairs) {
+__device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, uint32_t n, uint32_t pairs,
+                                                uint32_t spt) {
   constexpr uint32_t kLabelChunk = (RW == 1u && S >= 8u) ? 8u : 4u;
   auto row = [&](uint32_t i, uint32_t k) -> uint64_t {
     return M.nbr_lds ? reinterpret_cast<const TAMC_AS3 uint64_t *>(M.nbr_lds)[RW * i + k]
@@ -601,7 +604,7 @@ __device__ __forceinline__ void icm_label_pairs(Ctx &c, const IcmMem<kLds> &M, u
       int hooked = 0;
       // sites kLabelChunk at a time, their rows requested together
 #pragma unroll 1
-      for (uint32_t s0 = 0; s0 < S; s0 += kLabelChunk) {
+      for (uint32_t s0 = 0; s0 < spt; s0 += kLabelChunk) {
         uint64_t nb[kLabelChunk][RW];
 #pragma unroll
         for (uint32_t t = 0; t < kLabelChunk; ++t) {
@@ -798,7 +801,7 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
       break;
     }
   }
-  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs);
+  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs, S);
 }
 
 // ICM step (c) over compact tiles: the instance's sites are grouped into tiles of at most 64 that
@@ -895,7 +898,7 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
       break;
     }
   }
-  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs);
+  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs, S);
 }
 
 // ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster
@@ -932,12 +935,20 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   } else if (D > 4u && D <= 8u && n <= 16u * c.nthreads) {
     icm_grow_registers<16, 2>(c, M, n, D);
   } else {
-    uint32_t *any_slots = c.misc + 12;
-    if (c.tid < 2u) any_slots[c.tid] = 0u;
+    // sites in memory, any number of them.  Pass k ORs the pairs that gained sites into
+    // misc[8 + k % 3] (the slot of pass k + 2 is cleared after the barrier).  On graphs of at most
+    // four couplings per site (two-dimensional lattices, where clusters near percolation are long
+    // and take hundreds of passes, each a walk over every site in memory), the pairs that still
+    // grow after kLabelAfterPasses passes, if few, are finished by component labelling, which
+    // needs the 16-bit neighbour lists (and 16-bit site labels).  Wider rows (three-dimensional
+    // lattices) finish sooner by growth alone.
+    if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
     __syncthreads();
+    uint32_t label_pairs = 0u;
+    const bool can_label = D != 0u && D <= 4u && n < kNoLabel;
     for (uint32_t rnd = 0;; ++rnd) {
       if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
-      int changed = 0;
+      uint32_t gain = 0u;
       for (uint32_t rep = 0; rep < 2u; ++rep)
       for (uint32_t i = c.tid; i < n; i += c.nthreads) {
         const uint32_t q = M.qw[i];
@@ -967,10 +978,24 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
         const uint32_t nc = cur | (q & nb);
         if (nc != cur) {
           M.cw[i] = nc;
-          changed = 1;
+          gain |= nc ^ cur;
         }
       }
-      if (!block_any(c.tid, c.lane, any_slots, rnd, changed != 0)) break;
+      const uint32_t g = wave_or(gain);
+      uint32_t *slot = c.misc + 8u + rnd % 3u;
+      if (c.lane == 0u && g) atomicOr(slot, g);
+      __syncthreads();
+      const uint32_t grew = *slot;
+      if (c.tid == 0u) c.misc[8u + (rnd + 2u) % 3u] = 0u;
+      if (grew == 0u) break;
+      if (can_label && rnd + 1u >= kLabelAfterPasses && (uint32_t)__popc(grew) <= kMaxLabelPairs) {
+        label_pairs = grew;
+        break;
+      }
+    }
+    if (label_pairs) {
+      const uint32_t spt = (n + c.nthreads - 1u) / c.nthreads;
+      icm_label_pairs<16, 1>(c, M, n, label_pairs, (spt + 7u) & ~7u);
     }
   }
 }
