@@ -64,9 +64,12 @@ __device__ __forceinline__ float sa_sign(uint32_t w, uint32_t up) {
 // Row staging: a site's row blocks are copied into a per-wave LDS buffer with one 16-byte vector
 // load per lane (16 blocks = 128 entries per chunk); the copy for the next site is loaded while
 // the current site is processed, so the row is read from LDS with no memory latency in the way.
+// The buffer holds the chunk twice, once per half-wave, with each neighbour offset already turned
+// into the LDS address of that half's word of the neighbour (planes in LDS): a neighbour read then
+// needs no address arithmetic.
 typedef uint32_t W4 __attribute__((ext_vector_type(4)));  // four words moved as one 16-byte access
 constexpr uint32_t kRowChunkBlocks = 16;
-constexpr uint32_t kRowBufWords = kRowChunkBlocks * 16u;
+constexpr uint32_t kRowBufWords = 2u * kRowChunkBlocks * 16u;
 // LDS words of one wave: the plane words rounded up to 16 bytes, then the row buffer
 __host__ __device__ __forceinline__ uint32_t sa_plane_words(uint32_t n) { return (2u * n + 3u) & ~3u; }
 __host__ __device__ __forceinline__ uint32_t sa_wave_words(uint32_t n) { return sa_plane_words(n) + kRowBufWords; }
@@ -87,7 +90,41 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *lds)
   PW *sp = kG ? (PW *)X.spins_out : (PW *)lds;
   // this lane's half of every plane word: byte offsets are 8 * site + 4 * half
   PC *spl = (PC *)sp + half * 4u;
+  // the row buffer: copy h (W4 64h .. 64h + 63) for half-wave h; with the planes in LDS its
+  // offsets are absolute LDS byte addresses of that half's words
   TAMC_AS3 W4 *rb = (TAMC_AS3 W4 *)(lds + (kG ? 0u : sa_plane_words(n)));
+  const TAMC_AS3 W4 *rbh = rb + half * 64u;
+  const uint32_t base0 = kG ? 0u : (uint32_t)(uintptr_t)lds;
+  // lane L holds W4 L of a chunk (block L / 4, part L % 4; parts 0 and 1 are neighbour offsets)
+  auto stage = [&](W4 v) {
+    W4 a = v, b = v;
+    if (!kG && (lane & 2u) == 0u) {
+      a += base0;
+      b += base0 + 4u;
+    }
+    rb[lane] = a;
+    rb[64u + lane] = b;
+  };
+  // the word of this lane's half at a neighbour offset from the row buffer
+  auto word = [&](uint32_t off) -> uint32_t {
+    if constexpr (kG) return *(const PW *)(spl + off);
+    else return *(const TAMC_AS3 uint32_t *)(uintptr_t)off;
+  };
+  // the 32 entries of blocks b .. b + 3 of the row buffer
+  auto load_batch = [&](uint32_t b, uint32_t *off, uint32_t *jb) {
+#pragma unroll
+    for (uint32_t u = 0; u < 4u; ++u) {
+      const W4 o0 = rbh[4u * (b + u)], o1 = rbh[4u * (b + u) + 1u];
+      const W4 j0 = rbh[4u * (b + u) + 2u], j1 = rbh[4u * (b + u) + 3u];
+#pragma unroll
+      for (uint32_t e = 0; e < 4u; ++e) {
+        off[8u * u + e] = o0[e];
+        off[8u * u + 4u + e] = o1[e];
+        jb[8u * u + e] = j0[e];
+        jb[8u * u + 4u + e] = j1[e];
+      }
+    }
+  };
   const W4 *ginfo = (const W4 *)X.info;
   const W4 *gblk = (const W4 *)X.blocks;  // block b = W4 4b .. 4b+3
   auto put = [&](uint32_t i, uint64_t m) {
@@ -115,7 +152,7 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *lds)
   // pipeline: site i is processed from rb while the next site's first chunk and the info of the
   // site after it are in flight
   W4 inf_c = ginfo[0], inf_n = ginfo[n > 1u ? 1u : 0u];
-  rb[lane] = gblk[(size_t)inf_c.x * 4u + lane];
+  stage(gblk[(size_t)inf_c.x * 4u + lane]);
   for (uint32_t k = X.k0; k < X.k1; ++k) {
     const float beta = sa_c4(X.betas)[k];
     for (uint32_t q = 0; q < nq; ++q) {
@@ -132,38 +169,44 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *lds)
         // the reference starts from 0.0 and adds the bias (src/ising.rs:468-480)
         float d = __fadd_rn(0.0f, __uint_as_float(__builtin_amdgcn_readfirstlane(inf_c.z)));
         for (uint32_t c0 = 0; c0 < nb; c0 += kRowChunkBlocks) {
-          if (c0 > 0u) rb[lane] = gblk[((size_t)b0 + c0) * 4u + lane];  // rows beyond one chunk
+          if (c0 > 0u) stage(gblk[((size_t)b0 + c0) * 4u + lane]);  // rows beyond one chunk
           const uint32_t cb = (nb - c0 < kRowChunkBlocks) ? nb - c0 : kRowChunkBlocks;
-          // four blocks at a time: their 32 entries, then their 32 neighbour words are read
-          // together, then summed in row order; a remainder of one to three blocks goes singly
+          // four blocks at a time: their 32 neighbour words are read together and summed in row
+          // order, while the entries of the next four blocks are read (buffers A and B in turn; a
+          // read ahead past the chunk's last batch is clamped to blocks 12 .. 15 of the buffer and
+          // not used); a remainder of one to three blocks goes singly
           uint32_t b = 0;
-          for (; b + 4u <= cb; b += 4u) {
-            uint32_t off[32], jb[32], w[32];
+          if (cb >= 4u) {
+            uint32_t oA[32], jA[32], oB[32], jB[32], w[32];
+            load_batch(0u, oA, jA);
+            for (; b + 8u <= cb; b += 8u) {
 #pragma unroll
-            for (uint32_t u = 0; u < 4u; ++u) {
-              const W4 o0 = rb[4u * (b + u)], o1 = rb[4u * (b + u) + 1u];
-              const W4 j0 = rb[4u * (b + u) + 2u], j1 = rb[4u * (b + u) + 3u];
+              for (int e = 0; e < 32; ++e) w[e] = word(oA[e]);
+              load_batch(b + 4u, oB, jB);
+              // d += J * s_j in row order; J * (+-1.0) is exact, so each fma rounds like the add
 #pragma unroll
-              for (uint32_t e = 0; e < 4u; ++e) {
-                off[8u * u + e] = o0[e];
-                off[8u * u + 4u + e] = o1[e];
-                jb[8u * u + e] = j0[e];
-                jb[8u * u + 4u + e] = j1[e];
-              }
+              for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jA[e]), sa_sign(w[e], up), d);
+#pragma unroll
+              for (int e = 0; e < 32; ++e) w[e] = word(oB[e]);
+              load_batch(b + 8u < kRowChunkBlocks - 4u ? b + 8u : kRowChunkBlocks - 4u, oA, jA);
+#pragma unroll
+              for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jB[e]), sa_sign(w[e], up), d);
             }
+            if (b + 4u <= cb) {
 #pragma unroll
-            for (int e = 0; e < 32; ++e) w[e] = *(const PW *)(spl + off[e]);
-            // d += J * s_j in row order; J * (+-1.0) is exact, so each fma rounds like the add
+              for (int e = 0; e < 32; ++e) w[e] = word(oA[e]);
 #pragma unroll
-            for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jb[e]), sa_sign(w[e], up), d);
+              for (int e = 0; e < 32; ++e) d = __fmaf_rn(__uint_as_float(jA[e]), sa_sign(w[e], up), d);
+              b += 4u;
+            }
           }
           for (; b < cb; ++b) {
-            const W4 o0 = rb[4u * b], o1 = rb[4u * b + 1u], j0 = rb[4u * b + 2u], j1 = rb[4u * b + 3u];
+            const W4 o0 = rbh[4u * b], o1 = rbh[4u * b + 1u], j0 = rbh[4u * b + 2u], j1 = rbh[4u * b + 3u];
             uint32_t w[8];
 #pragma unroll
             for (uint32_t e = 0; e < 4u; ++e) {
-              w[e] = *(const PW *)(spl + o0[e]);
-              w[4u + e] = *(const PW *)(spl + o1[e]);
+              w[e] = word(o0[e]);
+              w[4u + e] = word(o1[e]);
             }
 #pragma unroll
             for (uint32_t e = 0; e < 4u; ++e) d = __fmaf_rn(__uint_as_float(j0[e]), sa_sign(w[e], up), d);
@@ -177,7 +220,7 @@ __device__ __forceinline__ void sa_wave(const SaTask &X, TAMC_AS3 uint32_t *lds)
         if (!acc) acc = sa_accept(-beta * d, us[s]);
         acc = acc && valid;
         put(i, __builtin_amdgcn_ballot_w64((bi ^ (acc ? 1u : 0u)) != 0u));
-        rb[lane] = chunk_n;  // after this site's reads of the buffer: LDS operations stay in order
+        stage(chunk_n);  // after this site's reads of the buffer: LDS operations stay in order
         inf_c = inf_n;
         inf_n = inf_nn;
       }
@@ -439,7 +482,7 @@ tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params 
   const tamc_sa_params &P = *params;
   if (tamc_status st = sa_validate(inst, P)) return st;
   if (!sa_wave_lds_words(inst->n))
-    return fail(TAMC_E_UNSUPPORTED, "the allocation-free SA keeps its planes in LDS (n <= 20352)");
+    return fail(TAMC_E_UNSUPPORTED, "the allocation-free SA keeps its planes in LDS (n <= 20224)");
   if ((size_t)P.num_betas * 4u > kNoAllocScratchBytes)
     return fail(TAMC_E_INVALID_ARG, "schedule longer than the allocation-free scratch holds (131072 betas)");
   const uint32_t n = inst->n;

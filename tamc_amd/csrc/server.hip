@@ -807,7 +807,7 @@ tamc_status tamc_server_submit_sa(tamc_server *srv, const tamc_instance *inst, c
   if (!srv || !ticket || !params) return fail(TAMC_E_INVALID_ARG, "null argument");
   if (tamc_status s = sa_validate(inst, *params)) return s;
   if (!sa_wave_lds_words(inst->n))
-    return fail(TAMC_E_UNSUPPORTED, "the server runs SA instances whose planes fit in LDS (n <= 20352); use "
+    return fail(TAMC_E_UNSUPPORTED, "the server runs SA instances whose planes fit in LDS (n <= 20224); use "
                                     "tamc_sa_run for larger instances");
   std::unique_ptr<Request> r(new Request());
   r->t_submit = Clock::now();
