@@ -721,7 +721,6 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
   // cleared after pass k's barrier (it is next written in pass k + 2)
   if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
   __syncthreads();
-  uint32_t label_pairs = 0u;
   for (uint32_t pass = 0;; ++pass) {
     if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
     uint32_t gain = 0u;
@@ -796,12 +795,7 @@ __device__ __forceinline__ void icm_grow_registers(Ctx &c, const IcmMem<kLds> &M
     const uint32_t grew = *slot;
     if (c.tid == 0u) c.misc[8u + (pass + 2u) % 3u] = 0u;
     if (grew == 0u) break;
-    if (pass + 1u >= kLabelAfterPasses && (uint32_t)__popc(grew) <= kMaxLabelPairs) {
-      label_pairs = grew;
-      break;
-    }
   }
-  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs, S);
 }
 
 // ICM step (c) over compact tiles: the instance's sites are grouped into tiles of at most 64 that
@@ -832,7 +826,6 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
   };
   if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
   __syncthreads();
-  uint32_t label_pairs = 0u;
   for (uint32_t pass = 0;; ++pass) {
     if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
     uint32_t gain = 0u;
@@ -893,12 +886,7 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
     const uint32_t grew = *slot;
     if (c.tid == 0u) c.misc[8u + (pass + 2u) % 3u] = 0u;
     if (grew == 0u) break;
-    if (pass + 1u >= kLabelAfterPasses && (uint32_t)__popc(grew) <= kMaxLabelPairs) {
-      label_pairs = grew;
-      break;
-    }
   }
-  if (label_pairs) icm_label_pairs<S, RW>(c, M, n, label_pairs, S);
 }
 
 // ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster

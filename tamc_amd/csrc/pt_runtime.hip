@@ -161,12 +161,14 @@ void want_buffers(SolvePlan &S, Arena &A, Arena *Ags, Arena *Ab) {
   }
 }
 
-// Default ICM growth order, from timings of both orders: compact tiles for rows wider than four
-// couplings (3D lattices: growth 25-35 % shorter) and for rows of four on up to 4096 sites (2D
-// lattices up to 64 x 64: 8-45 % shorter); strided sites otherwise (random 3-regular graphs, where
-// the strided order is 1.3-2x faster, and larger 2D lattices, whose rows come from HBM).
+// Default ICM growth order, from timings of both orders on one work group of 16 waves: compact
+// tiles for rows wider than four couplings (3D lattices: growth 25-35 % shorter); for rows of four
+// (2D lattices), tiles up to 1024 sites and above 4096, strided sites between (four sites per
+// thread, whose rows, q and cluster words stay in registers: the 64 x 64 lattice's sweep takes
+// 127 us against 191 us with tiles, while from 96 x 96 on, with rows in HBM, tiles take 714 us
+// against 1138 us); strided sites on random 3-regular graphs (1.3-2x faster there).
 static bool use_icm_tiles(bool have_tiles, uint32_t D, uint32_t n) {
-  return have_tiles && (D > 4u || (D == 4u && n <= 4096u));
+  return have_tiles && (D > 4u || (D == 4u && (n <= 1024u || n > 4096u)));
 }
 
 tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
