@@ -1142,7 +1142,8 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     // the spins are not read while clusters grow: park the planes in the group's spin array and
     // grow in the LDS they leave (q and cluster words take 8 bytes per site, the planes 8 G)
     const uint32_t nw = n * G * 2u;
-    for (uint32_t k = c.tid; k < nw; k += c.nthreads) X.spins[k] = c.sp[k];
+    uint32_t *park = X.g_park ? X.g_park : X.spins;
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) park[k] = c.sp[k];
     __syncthreads();
     IcmMem<true> S;
     S.qw = (TAMC_AS3 uint32_t *)c.sp;
@@ -1159,7 +1160,7 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     __syncthreads();
     for (uint32_t i = c.tid; i < n; i += c.nthreads) M.cw[i] = S.cw[i];
     __syncthreads();
-    for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = X.spins[k];
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = park[k];
     __syncthreads();
   }
   lap(c, kPhIcmGrow);

@@ -290,12 +290,19 @@ static double sweep_cost_us(const SolvePlan &S, const GroupPlan &g, uint32_t K) 
   return metro / K + icm + 8.0 + (K > 1 ? 4.0 * (levels + chunks) : 0.0);
 }
 
-// Teams need a group whose ICM scratch is in LDS (members keep private copies), no QUBO self
+// Teams need a group whose spin planes are in LDS (every member holds the whole state; a group
+// whose ICM scratch is in global memory gives each member a private copy of it), no QUBO self
 // entries (their drift is per flip) and at least two colour classes: a member rewrites a class's
 // exchange words only after the next meeting, which every member reaches after reading them.
 static bool team_eligible(const SolvePlan &S, const GroupPlan &g) {
-  return !g.icm_global && !(g.g.flags & kFlagIcmOnly) && g.g.self_J == nullptr && S.n >= 64u &&
+  return !g.spins_global && !(g.g.flags & kFlagIcmOnly) && g.g.self_J == nullptr && S.n >= 64u &&
          S.inst->sched[S.P.sweep_order].n_classes >= 2u;
+}
+
+// 32-bit words of a team member's private ICM scratch: q words, cluster words, per-block counts,
+// then the parking area of the spin planes
+static size_t icm_scratch_words(const SolvePlan &S, const GroupPlan &g) {
+  return 2u * (size_t)S.n + (size_t)((S.n + 63u) / 64u) * icm_chunk(g.P) + (size_t)S.n * g.G * 2u;
 }
 
 struct TeamJob {
@@ -465,7 +472,7 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     for (auto &j : w) used |= j.members > 1u;
   if (!used || mk <= 0.0 || (!force && mk > 0.9 * base) || wgs.size() > capacity) return TAMC_OK;
   // team scratch: counters (64-byte aligned) and the spin-word / energy-sum box of each team
-  std::vector<size_t> ctr_off(ni, 0), box_off(ni, 0);
+  std::vector<size_t> ctr_off(ni, 0), box_off(ni, 0), scr_off(ni, 0);
   std::vector<uint32_t> members(ni, 1u);
   for (auto &w : wgs)
     for (auto &j : w) members[j.item] = j.members;
@@ -478,6 +485,13 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     // level words, energy sums, then two flip buffers of block words and per-pair sums
     words += team_box_slots(items[i].S->n, items[i].g->G, items[i].g->R, members[i]);
     words = (words + 7u) & ~(size_t)7u;
+    if (items[i].g->icm_global) {
+      // members 1 .. K-1 grow their clusters in private copies of the ICM scratch (member 0 uses the
+      // group's own): members run the unshared ICM steps at different times, and one member's
+      // clearing of the cluster words for the next sweep must not meet another's seeds
+      scr_off[i] = words;
+      words += ((members[i] - 1u) * icm_scratch_words(*items[i].S, *items[i].g) + 15u) / 16u * 8u;
+    }
     ++Lc.teams;
   }
   TAMC_HIP(Lc.d_team.alloc(words * 8u));
@@ -492,6 +506,14 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
         d.member = j.member;
         d.team_ctr = (uint32_t *)(tb + ctr_off[j.item]);
         d.team_box = tb + box_off[j.item];
+        if (j.member > 0u && items[j.item].g->icm_global) {
+          uint32_t *scr = (uint32_t *)(tb + scr_off[j.item]) +
+                          (size_t)(j.member - 1u) * icm_scratch_words(*items[j.item].S, *items[j.item].g);
+          d.g_qw = scr;
+          d.g_cw = scr + items[j.item].S->n;
+          d.g_blk = scr + 2u * (size_t)items[j.item].S->n;
+          d.g_park = d.g_blk + (size_t)((items[j.item].S->n + 63u) / 64u) * icm_chunk(items[j.item].g->P);
+        }
         if (j.member == 0u) Lc.team_ctr.push_back(ctr_off[j.item] * 2u);
       }
       Lc.desc.push_back(d);

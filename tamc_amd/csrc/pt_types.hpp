@@ -83,6 +83,11 @@ struct PtGroup {
   uint64_t *out_gs_states;   // [gs_cap][gs_words]
   // ---- ICM scratch when it does not fit in LDS ----
   uint32_t *g_qw, *g_cw, *g_blk;
+  // where the spin planes wait while ICM clusters grow in the LDS they leave: null for the group's
+  // spin array; a private area for team members other than the lead (the lead alone writes the
+  // group's array: plain stores of other members would sit in their XCDs' L2 caches and could be
+  // written back after the lead's final state)
+  uint32_t *g_park;
   // ---- kFlagIcmOnly: seed index per pair (into the ascending q<0 list) ----
   const uint32_t *icm_k;
   // ---- fixed-stride copy of coupling_vecs for rows of at most ell_D entries (0 = use CSR) ----

@@ -343,6 +343,36 @@ def test_batch_equals_single_and_plain(gpu):
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
 
 
+@pytest.mark.parametrize("K", [3, 16])
+def test_teams_with_icm_scratch_in_memory(gpu, K):
+    # groups too large for their ICM scratch in LDS (spin planes in LDS, q and cluster words in
+    # global memory, grown in the LDS the parked planes leave): each member keeps a private copy of
+    # the scratch; same results as one work group and as the oracle, also over several launches
+    cases = [("2d_pm", dict(L=100)), ("rr3", dict(n=12000))]
+    insts, oracles, params = [], [], []
+    for k, (kind, kw) in enumerate(cases):
+        oi, gi, _ = make_pair(kind, seed=90 + k, **kw)
+        insts.append(gi)
+        oracles.append(oi)
+        params.append(api.PtIcmParams(num_sweeps=8, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, 12),
+                                      lo_num_beta=6, icm=True, num_replica_chains=2, sample=2, sample_states=3,
+                                      sample_limiting=2, seed=900 + k))
+    alone, ast = api.run_pt_batch(insts, params, teams=1)
+    assert ast["teams"] == 0
+    team, tst = api.run_pt_batch(insts, params, teams=K)
+    assert tst["teams"] == 2 and tst["work_groups"] >= 2 * K
+    chunked, _ = api.run_pt_batch(insts, params, teams=K, sweeps_per_launch=3)
+    for k, (oi, p) in enumerate(zip(oracles, params)):
+        runner = api.PtIcmRunner(insts[k], p)
+        o = O.pt_run(oi, runner.betas, 8, num_chains=2, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx,
+                     icm=True, sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
+        assert o["error"] == 0
+        assert_same(team[k], o, exact_energy=True, ctx=(K, k))
+        for key in EXACT_KEYS + ["e", "gs_energies", "final_energies"]:
+            assert np.array_equal(alone[k][key], team[k][key]), (K, k, key)
+            assert np.array_equal(chunked[k][key], team[k][key]), (K, k, key)
+
+
 @pytest.mark.parametrize("K", [2, 4, 16])
 def test_teams_share_the_sweep(gpu, K):
     # a group's Metropolis sweep shared by K work groups that exchange spin words after every colour
