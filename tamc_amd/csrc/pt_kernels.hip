@@ -889,6 +889,149 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
   }
 }
 
+// ICM step (c) on the periodic L x L square lattice with row-major sites, L a multiple of 8: the
+// lattice is cut into 8 x 8 tiles, and for each tile and pair the pair's q and cluster bits become
+// 64-bit bitboards (bit 8 y + x <=> site (8 ty + y) L + 8 tx + x of tile (ty, tx)).  A tile's
+// bitboards take the place of its own q and cluster words: bitboard p of tile (ty, tx) is the two
+// words of row 8 ty + p / 4, columns 8 tx + 2 (p % 4) and + 1 (a tile has 64 words of each kind,
+// 32 pairs need 64).  A thread takes one (tile, pair) at a time and joins sites by shifts until
+// nothing changes, the four adjacent tiles' edge rows and columns coming in as a halo; passes over
+// all (tile, pair) repeat until one changes nothing, which reaches the same fixpoint as the other
+// orders.  Afterwards the cluster words are rebuilt from the bitboards (the q words are not read
+// again in the sweep's ICM).
+template <bool kLds>
+__device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, uint32_t L) {
+  using U32 = typename IcmMem<kLds>::U32;
+  using U64 = std::conditional_t<kLds, TAMC_AS3 uint64_t, TAMC_AS1 uint64_t>;
+  const uint32_t T = L / 8u, nt = T * T, P = c.P;
+  auto bq = [&](uint32_t t, uint32_t p) -> U64 * {
+    return reinterpret_cast<U64 *>(M.qw + (size_t)(8u * (t / T) + p / 4u) * L + 8u * (t % T) + 2u * (p % 4u));
+  };
+  auto bc = [&](uint32_t t, uint32_t p) -> U64 * {
+    return reinterpret_cast<U64 *>(M.cw + (size_t)(8u * (t / T) + p / 4u) * L + 8u * (t % T) + 2u * (p % 4u));
+  };
+  auto site = [&](uint32_t t, uint32_t l) { return (8u * (t / T) + l / 8u) * L + 8u * (t % T) + l % 8u; };
+  // words -> bitboards, a wave per tile (its reads come before its writes: LDS operations of a wave
+  // stay in order, and no other wave touches the tile)
+  for (uint32_t t = c.wave; t < nt; t += c.W) {
+    const uint32_t i = site(t, c.lane);
+    const uint32_t q = M.qw[i], cw = M.cw[i];
+    uint64_t myq = 0ull, myc = 0ull;
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint64_t vq = ballot(((q >> p) & 1u) != 0u), vc = ballot(((cw >> p) & 1u) != 0u);
+      if (c.lane == p) {
+        myq = vq;
+        myc = vc;
+      }
+    }
+    if (c.lane < P) {
+      *bq(t, c.lane) = myq;
+      *bc(t, c.lane) = myc;
+    }
+  }
+  constexpr uint64_t kCol0 = 0x0101010101010101ull, kCol7 = 0x8080808080808080ull;
+  if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
+  __syncthreads();
+  // thread tid takes pair pp = tid % P and a run of m consecutive tiles (raster order: a strip of a
+  // tile row), m = nt / S rounded up, S = nthreads / P threads per pair; a pass makes kLatRounds
+  // rounds over the run between barriers, forwards and backwards in turn, so that a cluster crosses
+  // the run, and what other threads wrote in earlier rounds, without a barrier
+  constexpr uint32_t kLatRounds = 2u;
+  const uint32_t S = c.nthreads / P, pp = c.tid % P, t0 = c.tid / P;
+  const uint32_t m = (nt + S - 1u) / S, tb = t0 * m, te = (t0 < S && tb < nt) ? (tb + m < nt ? tb + m : nt) : tb;
+  const uint32_t poff = (pp >> 2) * L + 2u * (pp & 3u);
+  auto b64 = [&](U32 *a, uint32_t off) -> U64 * { return reinterpret_cast<U64 *>(a + off); };
+  for (uint32_t pass = 0;; ++pass) {
+    if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
+    uint32_t gain = 0u;
+    for (uint32_t rep = 0; rep < kLatRounds; ++rep) {
+      const bool back = ((pass * kLatRounds + rep) & 1u) != 0u, first = pass == 0u && rep == 0u;
+      const uint32_t ts = back ? te - 1u : tb;
+      uint32_t ty = ts / T, tx = ts % T;
+      for (uint32_t k = tb; k < te; ++k) {
+        const uint32_t o = 8u * ty * L + 8u * tx + poff;  // this tile's bitboard of pair pp
+        const uint32_t txl = tx ? tx - 1u : T - 1u, txr = tx + 1u < T ? tx + 1u : 0u;
+        const uint32_t tyu = ty ? ty - 1u : T - 1u, tyd = ty + 1u < T ? ty + 1u : 0u;
+        const uint32_t cx = tx, cy = ty;
+        if (back) {
+          if (tx) {
+            --tx;
+          } else {
+            tx = T - 1u;
+            --ty;
+          }
+        } else if (++tx == T) {
+          tx = 0u;
+          ++ty;
+        }
+        const uint64_t q = *b64(M.qw, o), cl = *b64(M.cw, o);
+        if (q == cl) continue;  // every q<0 site of the tile joined (or there is none)
+        const uint64_t left = *b64(M.cw, 8u * cy * L + 8u * txl + poff), right = *b64(M.cw, 8u * cy * L + 8u * txr + poff);
+        const uint64_t up = *b64(M.cw, 8u * tyu * L + 8u * cx + poff), down = *b64(M.cw, 8u * tyd * L + 8u * cx + poff);
+        const uint64_t h = ((left >> 7) & kCol0) | ((right << 7) & kCol7) | (up >> 56) | (down << 56);
+        // joins along the rows eastwards and westwards, then along the columns both ways, each as an
+        // occluded fill (a run of q<0 sites joins in three shift steps), until nothing changes
+        uint64_t x = cl | (q & h);
+        // after the first round a tile's bits are closed under these joins (only its own thread
+        // writes them, at the end of a fill), so without new halo bits nothing changes
+        if (x == cl && !first) continue;
+        const uint64_t pe = q & ~kCol0, pw = q & ~kCol7;
+        for (uint32_t it = 0; it < 64u; ++it) {  // each round joins at least one more site
+          uint64_t y = x, r = pe;
+          y |= r & (y << 1);
+          r &= r << 1;
+          y |= r & (y << 2);
+          r &= r << 2;
+          y |= r & (y << 4);
+          r = pw;
+          y |= r & (y >> 1);
+          r &= r >> 1;
+          y |= r & (y >> 2);
+          r &= r >> 2;
+          y |= r & (y >> 4);
+          r = q;
+          y |= r & (y << 8);
+          r &= r << 8;
+          y |= r & (y << 16);
+          r &= r << 16;
+          y |= r & (y << 32);
+          r = q;
+          y |= r & (y >> 8);
+          r &= r >> 8;
+          y |= r & (y >> 16);
+          r &= r >> 16;
+          y |= r & (y >> 32);
+          if (y == x) break;
+          x = y;
+        }
+        if (x != cl) {
+          *b64(M.cw, o) = x;
+          gain |= 1u << pp;
+        }
+      }
+    }
+    const uint32_t g = wave_or(gain);
+    uint32_t *slot = c.misc + 8u + pass % 3u;
+    if (c.lane == 0u && g) atomicOr(slot, g);
+    __syncthreads();
+    const uint32_t grew = *slot;
+    if (c.tid == 0u) c.misc[8u + (pass + 2u) % 3u] = 0u;
+    if (grew == 0u) break;
+  }
+  // bitboards -> cluster words
+  for (uint32_t t = c.wave; t < nt; t += c.W) {
+    const uint64_t myc = c.lane < P ? *bc(t, c.lane) : 0ull;
+    uint32_t cw = 0u;
+    for (uint32_t p = 0; p < P; ++p) {
+      const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(myc >> 32), (int)p) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myc, (int)p);
+      cw |= (uint32_t)((b >> c.lane) & 1ull) << p;
+    }
+    M.cw[site(t, c.lane)] = cw;
+  }
+  __syncthreads();
+}
+
 // ICM step (c): grow every pair's cluster through its q<0 sites.  A site joins pair p's cluster
 // when its q bit p is set and a neighbour is in it.  Passes update in place until a pass changes
 // nothing; each pass makes several rounds over the sites between barriers (four when the sites
@@ -899,6 +1042,12 @@ template <bool kLds>
 __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t n = c.n, D = c.nbr ? X.ell_D : 0u;
+  if constexpr (kLds) {
+    if (X.lat_L != 0u) {  // the square lattice: bitboard growth
+      icm_grow_lattice(c, M, X.lat_L);
+      return;
+    }
+  }
   if ((c.flags & kFlagIcmTiles) && X.icm_tiles != nullptr && D != 0u && n <= 16u * c.nthreads) {
     const uint32_t S = n <= 4u * c.nthreads ? 4u : (n <= 8u * c.nthreads ? 8u : 16u);
     if (D <= 4u) {

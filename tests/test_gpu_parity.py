@@ -343,6 +343,25 @@ def test_batch_equals_single_and_plain(gpu):
     assert st["attempts"] == sum(p.num_sweeps * i.n * 8 * 2 for p, i in zip(params, insts))
 
 
+@pytest.mark.parametrize("kind,L,B,lo", [("2d_pm", 24, 48, 8), ("2d_gauss", 40, 20, 4), ("2d_pm", 64, 36, 4),
+                                         ("2d_pm", 128, 12, 6)])
+def test_square_lattice_bitboard_growth(gpu, kind, L, B, lo):
+    # periodic L x L lattices with L % 8 == 0 grow ICM clusters as 8 x 8 bitboards (one chunk of 32
+    # pairs and a partial one at B - lo = 40; ICM arrays in LDS, or in the LDS the parked spin
+    # planes leave at L = 128): the oracle's clusters, energies and samples
+    oi, gi, _ = make_pair(kind, seed=120 + L, L=L)
+    p = api.PtIcmParams(num_sweeps=6, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, B),
+                        lo_num_beta=lo, icm=True, num_replica_chains=2, sample=2, sample_states=3,
+                        sample_limiting=2, seed=1200 + L)
+    runner = api.PtIcmRunner(gi, p)
+    g = runner.run()
+    o = O.pt_run(oi, runner.betas, 6, num_chains=2, warmup_fraction=0.5, lo_beta_idx=runner.lo_beta_idx, icm=True,
+                 sample=2, sample_states=3, sample_limiting=2, rng_mode=O.RNG_PHILOX, seed=p.seed)
+    assert o["error"] == 0
+    assert g["icm_moves"] > 0
+    assert_same(g, o, exact_energy=True, ctx=(kind, L))
+
+
 @pytest.mark.parametrize("K", [3, 16])
 def test_teams_with_icm_scratch_in_memory(gpu, K):
     # groups too large for their ICM scratch in LDS (spin planes in LDS, q and cluster words in
