@@ -245,28 +245,38 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
       return nullptr;
     }
     D->ell_D = Dl;
-    // the periodic L x L square lattice with row-major sites, L a multiple of 8 from 24 on: ICM
-    // grows its clusters as 8 x 8 bitboards (on 16 x 16 the compact tiles are a little faster)
-    if (Dl == 4u && inst->n < 65536u) {
-      uint32_t L = (uint32_t)std::lround(std::sqrt((double)inst->n));
-      bool lat = L >= 24u && L % 8u == 0u && L * L == inst->n;
+    // periodic lattices with row-major sites: the L x L square lattice, L a multiple of 8 from 24
+    // on, and the L x L x L cubic lattice, L a multiple of 4 from 12 on: ICM grows their clusters
+    // as 64-site bitboards (on 16 x 16 and 8 x 8 x 8 the compact tiles are a little faster)
+    if ((Dl == 4u || Dl == 6u) && inst->n < 65536u) {
+      const uint32_t dim = Dl / 2u;
+      const uint32_t L = (uint32_t)std::lround(dim == 2u ? std::sqrt((double)inst->n) : std::cbrt((double)inst->n));
+      bool lat = dim == 2u ? (L >= 24u && L % 8u == 0u && L * L == inst->n) : (L >= 12u && L % 4u == 0u && L * L * L == inst->n);
       for (uint32_t i = 0; i < inst->n && lat; ++i) {
-        if (inst->row_ptr[i + 1] - inst->row_ptr[i] != 4u) {
+        if (inst->row_ptr[i + 1] - inst->row_ptr[i] != Dl) {
           lat = false;
           break;
         }
-        const uint32_t r = i / L, c = i % L;
-        const uint32_t want[4] = {r * L + (c + 1u) % L, r * L + (c + L - 1u) % L, ((r + 1u) % L) * L + c,
-                                  ((r + L - 1u) % L) * L + c};
-        bool seen[4] = {false, false, false, false};
+        // site i = (a, b, c) with c fastest (a = 0 on the square lattice); neighbours +-1 along
+        // each axis, periodic
+        const uint32_t c = i % L, b = (i / L) % L, a = i / (L * L);
+        const uint32_t sa = L * L, sb = L;
+        uint32_t want[6] = {i - c + (c + 1u) % L, i - c + (c + L - 1u) % L, i - b * sb + ((b + 1u) % L) * sb,
+                            i - b * sb + ((b + L - 1u) % L) * sb, 0u, 0u};
+        if (dim == 3u) {
+          want[4] = i - a * sa + ((a + 1u) % L) * sa;
+          want[5] = i - a * sa + ((a + L - 1u) % L) * sa;
+        }
+        bool seen[6] = {false, false, false, false, false, false};
         for (uint32_t e = inst->row_ptr[i]; e < inst->row_ptr[i + 1] && lat; ++e) {
           uint32_t k = 0;
-          while (k < 4u && (seen[k] || want[k] != inst->col[e])) ++k;
-          if (k == 4u) lat = false;
+          while (k < Dl && (seen[k] || want[k] != inst->col[e])) ++k;
+          if (k == Dl) lat = false;
           else seen[k] = true;
         }
       }
       D->lat_L = lat ? L : 0u;
+      D->lat_dim = lat ? dim : 0u;
     }
     if (Dl && inst->n < 65536u) {
       // compact tiles of at most 64 sites for ICM cluster growth: balls of the coupling graph grown

@@ -889,32 +889,40 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
   }
 }
 
-// ICM step (c) on the periodic L x L square lattice with row-major sites, L a multiple of 8: the
-// lattice is cut into 8 x 8 tiles, and for each tile and pair the pair's q and cluster bits become
-// 64-bit bitboards (bit 8 y + x <=> site (8 ty + y) L + 8 tx + x of tile (ty, tx)).  A tile's
-// bitboards take the place of its own q and cluster words: bitboard p of tile (ty, tx) is the two
-// words of row 8 ty + p / 4, columns 8 tx + 2 (p % 4) and + 1 (a tile has 64 words of each kind,
-// 32 pairs need 64).  A thread takes one (tile, pair) at a time and joins sites by shifts until
-// nothing changes, the four adjacent tiles' edge rows and columns coming in as a halo; passes over
-// all (tile, pair) repeat until one changes nothing, which reaches the same fixpoint as the other
-// orders.  Afterwards the cluster words are rebuilt from the bitboards (the q words are not read
-// again in the sweep's ICM).
-template <bool kLds>
+// ICM step (c) on periodic lattices with row-major sites: the L x L square lattice (L a multiple
+// of 8) cut into 8 x 8 tiles, or the L x L x L cubic lattice (L a multiple of 4) cut into 4 x 4 x 4
+// tiles.  For each tile and pair the pair's q and cluster bits become 64-bit bitboards: bit
+// 8 y + x of tile (ty, tx) is site (8 ty + y) L + 8 tx + x, bit 16 z + 4 y + x of tile (tz, ty, tx)
+// is site (4 tz + z) L^2 + (4 ty + y) L + 4 tx + x.  A tile's bitboards take the place of its own
+// q and cluster words (a tile has 64 words of each kind, 32 pairs need 64): bitboard p is the two
+// words at columns 2 (p % 4) and + 1 of the tile's row p / 4 (square), or at columns 2 (p % 2) and
+// + 1 of row p / 2, rows running y-major within z (cubic).  A thread takes one (tile, pair) at a
+// time and joins sites by occluded fills along each axis until nothing changes, the adjacent
+// tiles' faces coming in as a halo; passes over all (tile, pair) repeat until one changes nothing,
+// which reaches the same fixpoint as the other orders.  Afterwards the cluster words are rebuilt
+// from the bitboards (the q words are not read again in the sweep's ICM).
+template <int kDim, bool kLds>
 __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, uint32_t L) {
   using U32 = typename IcmMem<kLds>::U32;
   using U64 = std::conditional_t<kLds, TAMC_AS3 uint64_t, TAMC_AS1 uint64_t>;
-  const uint32_t T = L / 8u, nt = T * T, P = c.P;
-  auto bq = [&](uint32_t t, uint32_t p) -> U64 * {
-    return reinterpret_cast<U64 *>(M.qw + (size_t)(8u * (t / T) + p / 4u) * L + 8u * (t % T) + 2u * (p % 4u));
+  constexpr uint32_t E = kDim == 2 ? 8u : 4u;  // tile edge
+  const uint32_t T = L / E, L2 = L * L, nt = kDim == 2 ? T * T : T * T * T, P = c.P;
+  // word offset of the first site of tile (tz, ty, tx) (tz = 0 on the square lattice), of bitboard
+  // p within a tile, and of lane l's site within a tile
+  auto tile_off = [&](uint32_t tz, uint32_t ty, uint32_t tx) { return E * (tz * L2 + ty * L + tx); };
+  auto bb_off = [&](uint32_t p) {
+    return kDim == 2 ? (p >> 2) * L + 2u * (p & 3u) : (p >> 3) * L2 + ((p >> 1) & 3u) * L + 2u * (p & 1u);
   };
-  auto bc = [&](uint32_t t, uint32_t p) -> U64 * {
-    return reinterpret_cast<U64 *>(M.cw + (size_t)(8u * (t / T) + p / 4u) * L + 8u * (t % T) + 2u * (p % 4u));
+  auto lane_off = [&](uint32_t l) { return kDim == 2 ? (l >> 3) * L + (l & 7u) : (l >> 4) * L2 + ((l >> 2) & 3u) * L + (l & 3u); };
+  auto tile_of = [&](uint32_t t) {  // tile t in raster order -> its first site's word offset
+    const uint32_t tx = t % T, r = t / T;
+    return kDim == 2 ? tile_off(0u, r, tx) : tile_off(r / T, r % T, tx);
   };
-  auto site = [&](uint32_t t, uint32_t l) { return (8u * (t / T) + l / 8u) * L + 8u * (t % T) + l % 8u; };
+  auto b64 = [&](U32 *a, uint32_t off) -> U64 * { return reinterpret_cast<U64 *>(a + off); };
   // words -> bitboards, a wave per tile (its reads come before its writes: LDS operations of a wave
   // stay in order, and no other wave touches the tile)
   for (uint32_t t = c.wave; t < nt; t += c.W) {
-    const uint32_t i = site(t, c.lane);
+    const uint32_t o = tile_of(t), i = o + lane_off(c.lane);
     const uint32_t q = M.qw[i], cw = M.cw[i];
     uint64_t myq = 0ull, myc = 0ull;
     for (uint32_t p = 0; p < P; ++p) {
@@ -925,11 +933,10 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
       }
     }
     if (c.lane < P) {
-      *bq(t, c.lane) = myq;
-      *bc(t, c.lane) = myc;
+      *b64(M.qw, o + bb_off(c.lane)) = myq;
+      *b64(M.cw, o + bb_off(c.lane)) = myc;
     }
   }
-  constexpr uint64_t kCol0 = 0x0101010101010101ull, kCol7 = 0x8080808080808080ull;
   if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
   __syncthreads();
   // thread tid takes pair pp = tid % P and a run of m consecutive tiles (raster order: a strip of a
@@ -939,68 +946,117 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
   constexpr uint32_t kLatRounds = 2u;
   const uint32_t S = c.nthreads / P, pp = c.tid % P, t0 = c.tid / P;
   const uint32_t m = (nt + S - 1u) / S, tb = t0 * m, te = (t0 < S && tb < nt) ? (tb + m < nt ? tb + m : nt) : tb;
-  const uint32_t poff = (pp >> 2) * L + 2u * (pp & 3u);
-  auto b64 = [&](U32 *a, uint32_t off) -> U64 * { return reinterpret_cast<U64 *>(a + off); };
+  const uint32_t poff = bb_off(pp);
   for (uint32_t pass = 0;; ++pass) {
     if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
     uint32_t gain = 0u;
     for (uint32_t rep = 0; rep < kLatRounds; ++rep) {
       const bool back = ((pass * kLatRounds + rep) & 1u) != 0u, first = pass == 0u && rep == 0u;
       const uint32_t ts = back ? te - 1u : tb;
-      uint32_t ty = ts / T, tx = ts % T;
+      // the tile's coordinates, stepped along the run (tx fastest)
+      uint32_t tx = ts % T, ty = (ts / T) % T, tz = kDim == 2 ? 0u : ts / (T * T);
+      if (kDim == 2) ty = ts / T;
       for (uint32_t k = tb; k < te; ++k) {
-        const uint32_t o = 8u * ty * L + 8u * tx + poff;  // this tile's bitboard of pair pp
-        const uint32_t txl = tx ? tx - 1u : T - 1u, txr = tx + 1u < T ? tx + 1u : 0u;
-        const uint32_t tyu = ty ? ty - 1u : T - 1u, tyd = ty + 1u < T ? ty + 1u : 0u;
-        const uint32_t cx = tx, cy = ty;
+        const uint32_t cx = tx, cy = ty, cz = tz;
         if (back) {
           if (tx) {
             --tx;
           } else {
             tx = T - 1u;
-            --ty;
+            if (kDim == 2 || ty) {
+              --ty;
+            } else {
+              ty = T - 1u;
+              --tz;
+            }
           }
         } else if (++tx == T) {
           tx = 0u;
-          ++ty;
+          if (++ty == T && kDim == 3) {
+            ty = 0u;
+            ++tz;
+          }
         }
+        const uint32_t o = tile_off(cz, cy, cx) + poff;  // this tile's bitboard of pair pp
         const uint64_t q = *b64(M.qw, o), cl = *b64(M.cw, o);
         if (q == cl) continue;  // every q<0 site of the tile joined (or there is none)
-        const uint64_t left = *b64(M.cw, 8u * cy * L + 8u * txl + poff), right = *b64(M.cw, 8u * cy * L + 8u * txr + poff);
-        const uint64_t up = *b64(M.cw, 8u * tyu * L + 8u * cx + poff), down = *b64(M.cw, 8u * tyd * L + 8u * cx + poff);
-        const uint64_t h = ((left >> 7) & kCol0) | ((right << 7) & kCol7) | (up >> 56) | (down << 56);
-        // joins along the rows eastwards and westwards, then along the columns both ways, each as an
-        // occluded fill (a run of q<0 sites joins in three shift steps), until nothing changes
+        const uint32_t xl = cx ? cx - 1u : T - 1u, xr = cx + 1u < T ? cx + 1u : 0u;
+        const uint32_t yl = cy ? cy - 1u : T - 1u, yr = cy + 1u < T ? cy + 1u : 0u;
+        uint64_t h;
+        if constexpr (kDim == 2) {
+          constexpr uint64_t kX0 = 0x0101010101010101ull, kX7 = 0x8080808080808080ull;
+          h = ((*b64(M.cw, tile_off(0u, cy, xl) + poff) >> 7) & kX0) | ((*b64(M.cw, tile_off(0u, cy, xr) + poff) << 7) & kX7) |
+              (*b64(M.cw, tile_off(0u, yl, cx) + poff) >> 56) | (*b64(M.cw, tile_off(0u, yr, cx) + poff) << 56);
+        } else {
+          constexpr uint64_t kX0 = 0x1111111111111111ull, kX3 = 0x8888888888888888ull;
+          constexpr uint64_t kY0 = 0x000F000F000F000Full, kY3 = 0xF000F000F000F000ull;
+          const uint32_t zl = cz ? cz - 1u : T - 1u, zr = cz + 1u < T ? cz + 1u : 0u;
+          h = ((*b64(M.cw, tile_off(cz, cy, xl) + poff) >> 3) & kX0) | ((*b64(M.cw, tile_off(cz, cy, xr) + poff) << 3) & kX3) |
+              ((*b64(M.cw, tile_off(cz, yl, cx) + poff) >> 12) & kY0) | ((*b64(M.cw, tile_off(cz, yr, cx) + poff) << 12) & kY3) |
+              (*b64(M.cw, tile_off(zl, cy, cx) + poff) >> 48) | (*b64(M.cw, tile_off(zr, cy, cx) + poff) << 48);
+        }
         uint64_t x = cl | (q & h);
         // after the first round a tile's bits are closed under these joins (only its own thread
         // writes them, at the end of a fill), so without new halo bits nothing changes
         if (x == cl && !first) continue;
-        const uint64_t pe = q & ~kCol0, pw = q & ~kCol7;
         for (uint32_t it = 0; it < 64u; ++it) {  // each round joins at least one more site
-          uint64_t y = x, r = pe;
-          y |= r & (y << 1);
-          r &= r << 1;
-          y |= r & (y << 2);
-          r &= r << 2;
-          y |= r & (y << 4);
-          r = pw;
-          y |= r & (y >> 1);
-          r &= r >> 1;
-          y |= r & (y >> 2);
-          r &= r >> 2;
-          y |= r & (y >> 4);
-          r = q;
-          y |= r & (y << 8);
-          r &= r << 8;
-          y |= r & (y << 16);
-          r &= r << 16;
-          y |= r & (y << 32);
-          r = q;
-          y |= r & (y >> 8);
-          r &= r >> 8;
-          y |= r & (y >> 16);
-          r &= r >> 16;
-          y |= r & (y >> 32);
+          uint64_t y = x, r;
+          // along x both ways (a site at x = 0 does not take the row before it, one at the last x
+          // not the row after it), then along y and z: each an occluded fill of shift steps
+          if constexpr (kDim == 2) {
+            constexpr uint64_t kX0 = 0x0101010101010101ull, kX7 = 0x8080808080808080ull;
+            r = q & ~kX0;
+            y |= r & (y << 1);
+            r &= r << 1;
+            y |= r & (y << 2);
+            r &= r << 2;
+            y |= r & (y << 4);
+            r = q & ~kX7;
+            y |= r & (y >> 1);
+            r &= r >> 1;
+            y |= r & (y >> 2);
+            r &= r >> 2;
+            y |= r & (y >> 4);
+            r = q;
+            y |= r & (y << 8);
+            r &= r << 8;
+            y |= r & (y << 16);
+            r &= r << 16;
+            y |= r & (y << 32);
+            r = q;
+            y |= r & (y >> 8);
+            r &= r >> 8;
+            y |= r & (y >> 16);
+            r &= r >> 16;
+            y |= r & (y >> 32);
+          } else {
+            constexpr uint64_t kX0 = 0x1111111111111111ull, kX3 = 0x8888888888888888ull;
+            constexpr uint64_t kY0 = 0x000F000F000F000Full, kY3 = 0xF000F000F000F000ull;
+            r = q & ~kX0;
+            y |= r & (y << 1);
+            r &= r << 1;
+            y |= r & (y << 2);
+            r = q & ~kX3;
+            y |= r & (y >> 1);
+            r &= r >> 1;
+            y |= r & (y >> 2);
+            r = q & ~kY0;
+            y |= r & (y << 4);
+            r &= r << 4;
+            y |= r & (y << 8);
+            r = q & ~kY3;
+            y |= r & (y >> 4);
+            r &= r >> 4;
+            y |= r & (y >> 8);
+            r = q;
+            y |= r & (y << 16);
+            r &= r << 16;
+            y |= r & (y << 32);
+            r = q;
+            y |= r & (y >> 16);
+            r &= r >> 16;
+            y |= r & (y >> 32);
+          }
           if (y == x) break;
           x = y;
         }
@@ -1020,14 +1076,15 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
   }
   // bitboards -> cluster words
   for (uint32_t t = c.wave; t < nt; t += c.W) {
-    const uint64_t myc = c.lane < P ? *bc(t, c.lane) : 0ull;
+    const uint32_t o = tile_of(t);
+    const uint64_t myc = c.lane < P ? *b64(M.cw, o + bb_off(c.lane)) : 0ull;
     uint32_t cw = 0u;
     for (uint32_t p = 0; p < P; ++p) {
       const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(myc >> 32), (int)p) << 32) |
                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myc, (int)p);
       cw |= (uint32_t)((b >> c.lane) & 1ull) << p;
     }
-    M.cw[site(t, c.lane)] = cw;
+    M.cw[o + lane_off(c.lane)] = cw;
   }
   __syncthreads();
 }
@@ -1043,8 +1100,9 @@ __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t n = c.n, D = c.nbr ? X.ell_D : 0u;
   if constexpr (kLds) {
-    if (X.lat_L != 0u) {  // the square lattice: bitboard growth
-      icm_grow_lattice(c, M, X.lat_L);
+    if (X.lat_L != 0u) {  // a square or cubic lattice: bitboard growth
+      if (X.lat_dim == 3u) icm_grow_lattice<3>(c, M, X.lat_L);
+      else icm_grow_lattice<2>(c, M, X.lat_L);
       return;
     }
   }

@@ -195,6 +195,7 @@ tamc_status fill_groups(SolvePlan &S, hipStream_t s, bool upload_betas) {
     X.icm_tiles = X.ell_col16 ? (const uint32_t *)D->icm_tiles.p : nullptr;
     X.icm_ntiles = X.icm_tiles ? D->icm_ntiles : 0u;
     X.lat_L = X.ell_col16 ? D->lat_L : 0u;
+    X.lat_dim = D->lat_dim;
     X.icm_trows = X.icm_tiles ? (const uint16_t *)D->icm_trows.p : nullptr;
     X.self_J = D->has_self ? (const float *)D->self_J.p : nullptr;
     X.offset = S.inst->offset;

@@ -107,7 +107,8 @@ struct DeviceInstance {
   DevBuf icm_tiles;           // [icm_ntiles][64] compact tiles of the coupling graph for ICM growth
   DevBuf icm_trows;           // [icm_ntiles * 64][nbr_stride(ell_D)] their 16-bit rows, in tile order
   uint32_t icm_ntiles = 0;
-  uint32_t lat_L = 0;         // L when the graph is the periodic L x L square lattice in row-major order
+  uint32_t lat_L = 0;         // L when the graph is a periodic square (lat_dim 2) or cubic (3) lattice
+  uint32_t lat_dim = 0;       // in row-major order
   bool has_self = false;
   uint32_t ell_D = 0;
   std::map<uint32_t, std::unique_ptr<DeviceSchedule>> sched;  // keyed by 2 * (words per site G) + order
