@@ -245,13 +245,14 @@ DeviceInstance *device_instance(const tamc_instance *cinst, uint32_t G, tamc_sta
       return nullptr;
     }
     D->ell_D = Dl;
-    // periodic lattices with row-major sites: the L x L square lattice, L a multiple of 8 from 24
-    // on, and the L x L x L cubic lattice, L a multiple of 4 from 12 on: ICM grows their clusters
-    // as 64-site bitboards (on 16 x 16 and 8 x 8 x 8 the compact tiles are a little faster)
+    // periodic lattices with row-major sites: the L x L square lattice from L = 24 on and the
+    // L x L x L cubic lattice with L a multiple of 4 from 12 on: ICM grows their clusters as 64-site
+    // bitboards (on smaller lattices, and on cubic ones with partial tiles, the compact tiles are as
+    // fast or faster)
     if ((Dl == 4u || Dl == 6u) && inst->n < 65536u) {
       const uint32_t dim = Dl / 2u;
       const uint32_t L = (uint32_t)std::lround(dim == 2u ? std::sqrt((double)inst->n) : std::cbrt((double)inst->n));
-      bool lat = dim == 2u ? (L >= 24u && L % 8u == 0u && L * L == inst->n) : (L >= 12u && L % 4u == 0u && L * L * L == inst->n);
+      bool lat = dim == 2u ? (L >= 24u && L * L == inst->n) : (L >= 12u && L % 4u == 0u && L * L * L == inst->n);
       for (uint32_t i = 0; i < inst->n && lat; ++i) {
         if (inst->row_ptr[i + 1] - inst->row_ptr[i] != Dl) {
           lat = false;

@@ -889,52 +889,81 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
   }
 }
 
-// ICM step (c) on periodic lattices with row-major sites: the L x L square lattice (L a multiple
-// of 8) cut into 8 x 8 tiles, or the L x L x L cubic lattice (L a multiple of 4) cut into 4 x 4 x 4
-// tiles.  For each tile and pair the pair's q and cluster bits become 64-bit bitboards: bit
-// 8 y + x of tile (ty, tx) is site (8 ty + y) L + 8 tx + x, bit 16 z + 4 y + x of tile (tz, ty, tx)
-// is site (4 tz + z) L^2 + (4 ty + y) L + 4 tx + x.  A tile's bitboards take the place of its own
-// q and cluster words (a tile has 64 words of each kind, 32 pairs need 64): bitboard p is the two
-// words at columns 2 (p % 4) and + 1 of the tile's row p / 4 (square), or at columns 2 (p % 2) and
-// + 1 of row p / 2, rows running y-major within z (cubic).  A thread takes one (tile, pair) at a
-// time and joins sites by occluded fills along each axis until nothing changes, the adjacent
-// tiles' faces coming in as a halo; passes over all (tile, pair) repeat until one changes nothing,
-// which reaches the same fixpoint as the other orders.  Afterwards the cluster words are rebuilt
-// from the bitboards (the q words are not read again in the sweep's ICM).
-template <int kDim, bool kLds>
-__device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, uint32_t L) {
-  using U32 = typename IcmMem<kLds>::U32;
-  using U64 = std::conditional_t<kLds, TAMC_AS3 uint64_t, TAMC_AS1 uint64_t>;
+// ICM step (c) on periodic lattices with row-major sites, the L x L square lattice cut into 8 x 8
+// tiles or the L x L x L cubic lattice cut into 4 x 4 x 4 tiles (the last tile along an axis holds
+// the remaining L - E (T - 1) sites, E the tile edge, T = ceil(L / E) tiles per axis).  For each
+// tile and pair the pair's q and cluster bits become 64-bit bitboards: bit 8 y + x of tile (ty, tx)
+// is site (8 ty + y) L + 8 tx + x, bit 16 z + 4 y + x of tile (tz, ty, tx) is site
+// (4 tz + z) L^2 + (4 ty + y) L + 4 tx + x; positions past the lattice's edge hold no q bit.  The
+// bitboards live in bb (LDS): q of tile t and pair p at 2 (t P + p), the cluster bits P T^dim
+// words later.  A thread takes one pair and a run of consecutive tiles and, per tile, joins sites
+// by occluded fills along each axis until nothing changes, the adjacent tiles' faces coming in as
+// a halo; passes over all (tile, pair) repeat until one changes nothing, which reaches the same
+// fixpoint as the other orders.  The cluster words are then rebuilt from the bitboards (the q
+// words are not read again in the sweep's ICM).
+template <int kDim>
+__host__ __device__ __forceinline__ uint32_t lattice_tiles(uint32_t L) {
+  constexpr uint32_t E = kDim == 2 ? 8u : 4u;
+  const uint32_t T = (L + E - 1u) / E;
+  return kDim == 2 ? T * T : T * T * T;
+}
+
+template <int kDim, class SrcU32>
+__device__ __forceinline__ void icm_grow_lattice(Ctx &c, SrcU32 *qw, SrcU32 *cw, uint32_t L,
+                                                 TAMC_AS3 uint64_t *bb) {
   constexpr uint32_t E = kDim == 2 ? 8u : 4u;  // tile edge
-  const uint32_t T = L / E, L2 = L * L, nt = kDim == 2 ? T * T : T * T * T, P = c.P;
-  // word offset of the first site of tile (tz, ty, tx) (tz = 0 on the square lattice), of bitboard
-  // p within a tile, and of lane l's site within a tile
-  auto tile_off = [&](uint32_t tz, uint32_t ty, uint32_t tx) { return E * (tz * L2 + ty * L + tx); };
-  auto bb_off = [&](uint32_t p) {
-    return kDim == 2 ? (p >> 2) * L + 2u * (p & 3u) : (p >> 3) * L2 + ((p >> 1) & 3u) * L + 2u * (p & 1u);
-  };
-  auto lane_off = [&](uint32_t l) { return kDim == 2 ? (l >> 3) * L + (l & 7u) : (l >> 4) * L2 + ((l >> 2) & 3u) * L + (l & 3u); };
-  auto tile_of = [&](uint32_t t) {  // tile t in raster order -> its first site's word offset
-    const uint32_t tx = t % T, r = t / T;
-    return kDim == 2 ? tile_off(0u, r, tx) : tile_off(r / T, r % T, tx);
-  };
-  auto b64 = [&](U32 *a, uint32_t off) -> U64 * { return reinterpret_cast<U64 *>(a + off); };
-  // words -> bitboards, a wave per tile (its reads come before its writes: LDS operations of a wave
-  // stay in order, and no other wave touches the tile)
-  for (uint32_t t = c.wave; t < nt; t += c.W) {
-    const uint32_t o = tile_of(t), i = o + lane_off(c.lane);
-    const uint32_t q = M.qw[i], cw = M.cw[i];
-    uint64_t myq = 0ull, myc = 0ull;
-    for (uint32_t p = 0; p < P; ++p) {
-      const uint64_t vq = ballot(((q >> p) & 1u) != 0u), vc = ballot(((cw >> p) & 1u) != 0u);
-      if (c.lane == p) {
-        myq = vq;
-        myc = vc;
-      }
+  const uint32_t T = (L + E - 1u) / E, nt = lattice_tiles<kDim>(L), P = c.P, last = L - E * (T - 1u);
+  const uint32_t L2 = L * L;
+  TAMC_AS3 uint64_t *bq = bb, *bcl = bb + (size_t)nt * P;
+  auto ext = [&](uint32_t tc) { return tc + 1u == T ? last : E; };  // sites of a tile along an axis
+  // lane l's site in tile (tz, ty, tx), or kNone past the lattice's edge
+  auto lane_site = [&](uint32_t tz, uint32_t ty, uint32_t tx, uint32_t l) -> uint32_t {
+    if constexpr (kDim == 2) {
+      const uint32_t y = l >> 3, x = l & 7u;
+      return (y < ext(ty) && x < ext(tx)) ? (E * ty + y) * L + E * tx + x : kNone;
+    } else {
+      const uint32_t z = l >> 4, y = (l >> 2) & 3u, x = l & 3u;
+      return (z < ext(tz) && y < ext(ty) && x < ext(tx)) ? (E * tz + z) * L2 + (E * ty + y) * L + E * tx + x : kNone;
     }
-    if (c.lane < P) {
-      *b64(M.qw, o + bb_off(c.lane)) = myq;
-      *b64(M.cw, o + bb_off(c.lane)) = myc;
+  };
+  auto coords = [&](uint32_t t, uint32_t &tz, uint32_t &ty, uint32_t &tx) {
+    tx = t % T;
+    ty = (t / T) % T;
+    tz = kDim == 2 ? 0u : t / (T * T);
+  };
+  // words -> bitboards, a wave per tile, four tiles' words requested together (from HBM they take
+  // a round trip each)
+  auto words = [&](uint32_t t, uint32_t &q, uint32_t &cl) {
+    q = cl = 0u;
+    if (t >= nt) return;
+    uint32_t tz, ty, tx;
+    coords(t, tz, ty, tx);
+    const uint32_t i = lane_site(tz, ty, tx, c.lane);
+    if (i != kNone) {
+      q = qw[i];
+      cl = cw[i];
+    }
+  };
+  for (uint32_t t4 = c.wave; t4 < nt; t4 += 4u * c.W) {
+    uint32_t q4[4], c4w[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) words(t4 + j * c.W, q4[j], c4w[j]);
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+      const uint32_t t = t4 + j * c.W;
+      if (t >= nt) break;
+      uint64_t myq = 0ull, myc = 0ull;
+      for (uint32_t p = 0; p < P; ++p) {
+        const uint64_t vq = ballot(((q4[j] >> p) & 1u) != 0u), vc = ballot(((c4w[j] >> p) & 1u) != 0u);
+        if (c.lane == p) {
+          myq = vq;
+          myc = vc;
+        }
+      }
+      if (c.lane < P) {
+        bq[(size_t)t * P + c.lane] = myq;
+        bcl[(size_t)t * P + c.lane] = myc;
+      }
     }
   }
   if (c.tid < 3u) c.misc[8u + c.tid] = 0u;
@@ -946,19 +975,19 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
   constexpr uint32_t kLatRounds = 2u;
   const uint32_t S = c.nthreads / P, pp = c.tid % P, t0 = c.tid / P;
   const uint32_t m = (nt + S - 1u) / S, tb = t0 * m, te = (t0 < S && tb < nt) ? (tb + m < nt ? tb + m : nt) : tb;
-  const uint32_t poff = bb_off(pp);
+  const uint32_t sx = P, sy = T * P, sz = T * T * P;
   for (uint32_t pass = 0;; ++pass) {
     if (c.tid == 0u) c.ticks[kPhIcmPasses] += 1u;
     uint32_t gain = 0u;
     for (uint32_t rep = 0; rep < kLatRounds; ++rep) {
       const bool back = ((pass * kLatRounds + rep) & 1u) != 0u, first = pass == 0u && rep == 0u;
-      const uint32_t ts = back ? te - 1u : tb;
-      // the tile's coordinates, stepped along the run (tx fastest)
-      uint32_t tx = ts % T, ty = (ts / T) % T, tz = kDim == 2 ? 0u : ts / (T * T);
-      if (kDim == 2) ty = ts / T;
-      for (uint32_t k = tb; k < te; ++k) {
+      uint32_t tz, ty, tx;
+      coords(back ? te - 1u : tb, tz, ty, tx);
+      // this tile's bitboards of pair pp: index (tile in raster order) P + pp, one tile on per step
+      uint32_t o = (back ? te - 1u : tb) * P + pp;
+      for (uint32_t k = tb; k < te; ++k, o = back ? o - P : o + P) {
         const uint32_t cx = tx, cy = ty, cz = tz;
-        if (back) {
+        if (back) {  // step to the previous tile in raster order
           if (tx) {
             --tx;
           } else {
@@ -977,23 +1006,26 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
             ++tz;
           }
         }
-        const uint32_t o = tile_off(cz, cy, cx) + poff;  // this tile's bitboard of pair pp
-        const uint64_t q = *b64(M.qw, o), cl = *b64(M.cw, o);
+        const uint64_t q = bq[o], cl = bcl[o];
         if (q == cl) continue;  // every q<0 site of the tile joined (or there is none)
-        const uint32_t xl = cx ? cx - 1u : T - 1u, xr = cx + 1u < T ? cx + 1u : 0u;
-        const uint32_t yl = cy ? cy - 1u : T - 1u, yr = cy + 1u < T ? cy + 1u : 0u;
+        // the adjacent tiles' bitboards (index steps sx, sy, sz along the axes, periodic) and their
+        // extents; the halo: their last and first layers along each axis, moved to this tile's
+        // first and last layers
+        const uint32_t ol = cx ? o - sx : o + (T - 1u) * sx, or_ = cx + 1u < T ? o + sx : o - (T - 1u) * sx;
+        const uint32_t ou = cy ? o - sy : o + (T - 1u) * sy, od = cy + 1u < T ? o + sy : o - (T - 1u) * sy;
+        const uint32_t el = cx ? E : last, eu = cy ? E : last;  // the tile before along x / y
         uint64_t h;
         if constexpr (kDim == 2) {
-          constexpr uint64_t kX0 = 0x0101010101010101ull, kX7 = 0x8080808080808080ull;
-          h = ((*b64(M.cw, tile_off(0u, cy, xl) + poff) >> 7) & kX0) | ((*b64(M.cw, tile_off(0u, cy, xr) + poff) << 7) & kX7) |
-              (*b64(M.cw, tile_off(0u, yl, cx) + poff) >> 56) | (*b64(M.cw, tile_off(0u, yr, cx) + poff) << 56);
+          constexpr uint64_t kX0 = 0x0101010101010101ull;
+          h = ((bcl[ol] >> (el - 1u)) & kX0) | ((bcl[or_] & kX0) << (ext(cx) - 1u)) |
+              ((bcl[ou] >> (8u * (eu - 1u))) & 0xFFull) | ((bcl[od] & 0xFFull) << (8u * (ext(cy) - 1u)));
         } else {
-          constexpr uint64_t kX0 = 0x1111111111111111ull, kX3 = 0x8888888888888888ull;
-          constexpr uint64_t kY0 = 0x000F000F000F000Full, kY3 = 0xF000F000F000F000ull;
-          const uint32_t zl = cz ? cz - 1u : T - 1u, zr = cz + 1u < T ? cz + 1u : 0u;
-          h = ((*b64(M.cw, tile_off(cz, cy, xl) + poff) >> 3) & kX0) | ((*b64(M.cw, tile_off(cz, cy, xr) + poff) << 3) & kX3) |
-              ((*b64(M.cw, tile_off(cz, yl, cx) + poff) >> 12) & kY0) | ((*b64(M.cw, tile_off(cz, yr, cx) + poff) << 12) & kY3) |
-              (*b64(M.cw, tile_off(zl, cy, cx) + poff) >> 48) | (*b64(M.cw, tile_off(zr, cy, cx) + poff) << 48);
+          constexpr uint64_t kX0 = 0x1111111111111111ull, kY0 = 0x000F000F000F000Full;
+          const uint32_t oz = cz ? o - sz : o + (T - 1u) * sz, ozd = cz + 1u < T ? o + sz : o - (T - 1u) * sz;
+          const uint32_t ez = cz ? E : last;
+          h = ((bcl[ol] >> (el - 1u)) & kX0) | ((bcl[or_] & kX0) << (ext(cx) - 1u)) |
+              ((bcl[ou] >> (4u * (eu - 1u))) & kY0) | ((bcl[od] & kY0) << (4u * (ext(cy) - 1u))) |
+              ((bcl[oz] >> (16u * (ez - 1u))) & 0xFFFFull) | ((bcl[ozd] & 0xFFFFull) << (16u * (ext(cz) - 1u)));
         }
         uint64_t x = cl | (q & h);
         // after the first round a tile's bits are closed under these joins (only its own thread
@@ -1002,7 +1034,8 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
         for (uint32_t it = 0; it < 64u; ++it) {  // each round joins at least one more site
           uint64_t y = x, r;
           // along x both ways (a site at x = 0 does not take the row before it, one at the last x
-          // not the row after it), then along y and z: each an occluded fill of shift steps
+          // not the row after it), then along y and z: each an occluded fill of shift steps; no
+          // q bit lies past the lattice's edge, so nothing joins there
           if constexpr (kDim == 2) {
             constexpr uint64_t kX0 = 0x0101010101010101ull, kX7 = 0x8080808080808080ull;
             r = q & ~kX0;
@@ -1061,7 +1094,7 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
           x = y;
         }
         if (x != cl) {
-          *b64(M.cw, o) = x;
+          bcl[o] = x;
           gain |= 1u << pp;
         }
       }
@@ -1076,15 +1109,17 @@ __device__ __forceinline__ void icm_grow_lattice(Ctx &c, const IcmMem<kLds> &M, 
   }
   // bitboards -> cluster words
   for (uint32_t t = c.wave; t < nt; t += c.W) {
-    const uint32_t o = tile_of(t);
-    const uint64_t myc = c.lane < P ? *b64(M.cw, o + bb_off(c.lane)) : 0ull;
-    uint32_t cw = 0u;
+    uint32_t tz, ty, tx;
+    coords(t, tz, ty, tx);
+    const uint64_t myc = c.lane < P ? bcl[(size_t)t * P + c.lane] : 0ull;
+    uint32_t w = 0u;
     for (uint32_t p = 0; p < P; ++p) {
       const uint64_t b = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(myc >> 32), (int)p) << 32) |
                          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)myc, (int)p);
-      cw |= (uint32_t)((b >> c.lane) & 1ull) << p;
+      w |= (uint32_t)((b >> c.lane) & 1ull) << p;
     }
-    M.cw[o + lane_off(c.lane)] = cw;
+    const uint32_t i = lane_site(tz, ty, tx, c.lane);
+    if (i != kNone) cw[i] = w;
   }
   __syncthreads();
 }
@@ -1099,13 +1134,6 @@ template <bool kLds>
 __device__ __forceinline__ void icm_grow(Ctx &c, const IcmMem<kLds> &M) {
   const TAMC_AS4 PtGroup &X = *c.X;
   const uint32_t n = c.n, D = c.nbr ? X.ell_D : 0u;
-  if constexpr (kLds) {
-    if (X.lat_L != 0u) {  // a square or cubic lattice: bitboard growth
-      if (X.lat_dim == 3u) icm_grow_lattice<3>(c, M, X.lat_L);
-      else icm_grow_lattice<2>(c, M, X.lat_L);
-      return;
-    }
-  }
   if ((c.flags & kFlagIcmTiles) && X.icm_tiles != nullptr && D != 0u && n <= 16u * c.nthreads) {
     const uint32_t S = n <= 4u * c.nthreads ? 4u : (n <= 8u * c.nthreads ? 8u : 16u);
     if (D <= 4u) {
@@ -1341,7 +1369,21 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
   }
   lap(c, kPhIcmSeed);
   __syncthreads();
-  if constexpr (kLds || kSpinsGlobal) {
+  // square and cubic lattices: bitboards in the LDS of the parked spin planes (which are not read
+  // while clusters grow), when they fit
+  const bool lattice = !kSpinsGlobal && X.lat_L != 0u &&
+                       4u * (X.lat_dim == 3u ? lattice_tiles<3>(X.lat_L) : lattice_tiles<2>(X.lat_L)) * P <= n * G * 2u;
+  if (lattice) {
+    const uint32_t nw = n * G * 2u;
+    uint32_t *park = X.g_park ? X.g_park : X.spins;
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) park[k] = c.sp[k];
+    __syncthreads();
+    TAMC_AS3 uint64_t *bb = reinterpret_cast<TAMC_AS3 uint64_t *>((TAMC_AS3 uint32_t *)c.sp);
+    if (X.lat_dim == 3u) icm_grow_lattice<3>(c, M.qw, M.cw, X.lat_L, bb);
+    else icm_grow_lattice<2>(c, M.qw, M.cw, X.lat_L, bb);
+    for (uint32_t k = c.tid; k < nw; k += c.nthreads) c.sp[k] = park[k];
+    __syncthreads();
+  } else if constexpr (kLds || kSpinsGlobal) {
     // (with the spin planes in global memory there is no LDS to borrow: the clusters grow in the
     // global scratch)
     icm_grow(c, M);

@@ -487,10 +487,11 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
     // level words, energy sums, then two flip buffers of block words and per-pair sums
     words += team_box_slots(items[i].S->n, items[i].g->G, items[i].g->R, members[i]);
     words = (words + 7u) & ~(size_t)7u;
-    if (items[i].g->icm_global) {
+    if (items[i].g->icm_global || Lc.order[i].lat_L != 0u) {
       // members 1 .. K-1 grow their clusters in private copies of the ICM scratch (member 0 uses the
       // group's own): members run the unshared ICM steps at different times, and one member's
-      // clearing of the cluster words for the next sweep must not meet another's seeds
+      // clearing of the cluster words for the next sweep must not meet another's seeds; and they
+      // park their spin planes (HBM scratch, or lattice growth) in private areas
       scr_off[i] = words;
       words += ((members[i] - 1u) * icm_scratch_words(*items[i].S, *items[i].g) + 15u) / 16u * 8u;
     }
@@ -508,7 +509,7 @@ static tamc_status prepare_launch(std::vector<SolvePlan> &solves, uint32_t waves
         d.member = j.member;
         d.team_ctr = (uint32_t *)(tb + ctr_off[j.item]);
         d.team_box = tb + box_off[j.item];
-        if (j.member > 0u && items[j.item].g->icm_global) {
+        if (j.member > 0u && (items[j.item].g->icm_global || Lc.order[j.item].lat_L != 0u)) {
           uint32_t *scr = (uint32_t *)(tb + scr_off[j.item]) +
                           (size_t)(j.member - 1u) * icm_scratch_words(*items[j.item].S, *items[j.item].g);
           d.g_qw = scr;

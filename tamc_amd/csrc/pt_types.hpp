@@ -95,8 +95,8 @@ struct PtGroup {
   const uint16_t *ell_col16; // 16-bit neighbour lists [n][nbr_stride(ell_D)] (n < 65536), for ICM
   const float *ell_J;        // [n][ell_D], padding entries are +0.0
   uint32_t ell_D, icm_ntiles;
-  uint32_t lat_L, lat_dim;   // a periodic lattice with row-major sites: L x L (lat_dim 2, L % 8 == 0) or
-                             // L x L x L (lat_dim 3, L % 4 == 0); lat_L = 0 for other graphs
+  uint32_t lat_L, lat_dim;   // a periodic lattice with row-major sites: L x L (lat_dim 2) or L x L x L
+                             // (lat_dim 3); lat_L = 0 for other graphs
   const uint32_t *icm_tiles; // [icm_ntiles][64] sites of each compact tile (kNone pads), for ICM
   const uint16_t *icm_trows; // [icm_ntiles * 64][nbr_stride(ell_D)] their neighbour rows, tile order
   // ---- QUBO self entries (src/ising.rs:308-310): J_ii per site, null when the instance has none.

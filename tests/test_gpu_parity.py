@@ -344,13 +344,14 @@ def test_batch_equals_single_and_plain(gpu):
 
 
 @pytest.mark.parametrize("kind,L,B,lo", [("2d_pm", 24, 48, 8), ("2d_gauss", 40, 20, 4), ("2d_pm", 64, 36, 4),
-                                         ("2d_pm", 128, 12, 6), ("3d_gauss", 12, 48, 8),
-                                         ("3d_pm", 16, 20, 4), ("3d_pm", 24, 12, 6)])
+                                         ("2d_pm", 128, 12, 6), ("2d_pm", 25, 48, 8), ("2d_gauss", 57, 20, 4),
+                                         ("2d_pm", 127, 12, 6), ("3d_gauss", 12, 48, 8), ("3d_pm", 16, 20, 4),
+                                         ("3d_pm", 24, 12, 6), ("3d_gauss", 20, 12, 6)])
 def test_lattice_bitboard_growth(gpu, kind, L, B, lo):
-    # periodic L x L lattices (L % 8 == 0, L >= 24) and L x L x L lattices (L % 4 == 0, L >= 12)
-    # grow ICM clusters as 64-site bitboards (one chunk of 32 pairs and a partial one at B - lo = 40; ICM
-    # arrays in LDS, or in the LDS the parked spin planes leave at L = 128 and L = 24): the
-    # oracle's clusters, energies and samples
+    # periodic L x L lattices (L >= 24; the last tiles along each axis partial unless 8 divides L)
+    # and L x L x L lattices (L % 4 == 0, L >= 12) grow ICM clusters as 64-site bitboards (one
+    # chunk of 32 pairs and a partial one at B - lo = 40; ICM arrays in LDS, or in HBM with the
+    # spin planes parked at the largest sizes): the oracle's clusters, energies and samples
     oi, gi, _ = make_pair(kind, seed=120 + L, L=L)
     p = api.PtIcmParams(num_sweeps=6, warmup_fraction=0.5, beta=api.BetaOptions.Geometric(0.2, 5.0, B),
                         lo_num_beta=lo, icm=True, num_replica_chains=2, sample=2, sample_states=3,
