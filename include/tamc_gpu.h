@@ -379,7 +379,8 @@ tamc_status tamc_server_poll_sa(tamc_server *srv, uint64_t ticket, int32_t wait_
  * every call on that GPU shares and which keeps the last call's contents.  The instance must
  * already be on the device (any earlier call that ran it uploads it), else TAMC_E_INVALID_ARG.
  * stream is a hipStream_t (null: the default stream); the work is queued on it and the call returns
- * without waiting.  Calls that overlap on one GPU race on the scratch. */
+ * without waiting.  Calls that overlap on one GPU race on the scratch.  The restarts' bit planes
+ * stay in LDS, so the instance may have at most 20224 sites (else TAMC_E_UNSUPPORTED). */
 tamc_status tamc_sa_run_noalloc(const tamc_instance *inst, const tamc_sa_params *params, float *energies,
                                 int8_t *states, void *stream);
 
