@@ -895,8 +895,8 @@ __device__ __forceinline__ void icm_grow_tiles(Ctx &c, const IcmMem<kLds> &M, ui
 // tile and pair the pair's q and cluster bits become 64-bit bitboards: bit 8 y + x of tile (ty, tx)
 // is site (8 ty + y) L + 8 tx + x, bit 16 z + 4 y + x of tile (tz, ty, tx) is site
 // (4 tz + z) L^2 + (4 ty + y) L + 4 tx + x; positions past the lattice's edge hold no q bit.  The
-// bitboards live in bb (LDS): q of tile t and pair p at 2 (t P + p), the cluster bits P T^dim
-// words later.  A thread takes one pair and a run of consecutive tiles and, per tile, joins sites
+// bitboards live in bb (LDS): q of tile t and pair p at word 2 (t P + p), the cluster bits
+// 2 P T^dim words later.  A thread takes one pair and a run of consecutive tiles and, per tile, joins sites
 // by occluded fills along each axis until nothing changes, the adjacent tiles' faces coming in as
 // a halo; passes over all (tile, pair) repeat until one changes nothing, which reaches the same
 // fixpoint as the other orders.  The cluster words are then rebuilt from the bitboards (the q
