@@ -1292,8 +1292,16 @@ __device__ __forceinline__ void icm_phase(Ctx &c, uint32_t t) {
     const bool valid = i < n;
     uint32_t q = 0;
     if (valid && G == 1u) {
+      // the pairs' replicas four at a time (one 16-byte LDS read each for a and b; entries past P
+      // are in bounds and unused), so a block waits for P / 4 LDS round trips instead of P
       const uint64_t w = *reinterpret_cast<const uint64_t *>(c.sp + 2u * i);
-      for (uint32_t p = 0; p < P; ++p) q |= (uint32_t)(((w >> c.pair_a[p]) ^ (w >> c.pair_b[p])) & 1ull) << p;
+      typedef uint32_t U4v __attribute__((ext_vector_type(4)));
+      for (uint32_t k = 0; 4u * k < P; ++k) {
+        const U4v av = reinterpret_cast<const U4v *>(c.pair_a)[k], bv = reinterpret_cast<const U4v *>(c.pair_b)[k];
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j)
+          if (4u * k + j < P) q |= (uint32_t)(((w >> av[j]) ^ (w >> bv[j])) & 1ull) << (4u * k + j);
+      }
     } else if (valid && tables) {
       for (uint32_t g = 0; g < G; ++g) {
         const uint32_t lo = c.sp[(i * G + g) * 2u], hi = c.sp[(i * G + g) * 2u + 1u];
